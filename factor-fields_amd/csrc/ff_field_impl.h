@@ -1,0 +1,692 @@
+// Field evaluation kernels: grid_mapping + multi-level basis gather +
+// coefficient gather (grid / VM) + coeff*basis + linear_mat decoder, forward
+// and hand-written backward, for gfx950 (wave64).
+//
+// Reference semantics: models/FactorFields.py:11-33 (grid_mapping),
+// :425-465 (get_coeff), :467-516 (get_basis), :523-533 (get_coding),
+// :113-159 (MLPMixer / linear_mat).
+//
+// Work decomposition: one lane per point.  Each level's corners are a
+// contiguous C-vector in the texel-major tables, so one point issues one
+// vector load per corner; interpolation weights stay in registers and are
+// recomputed in the backward pass instead of being stored.  The decoder runs
+// on the same lane with its weights read through the scalar cache.  Weight
+// gradients of the decoder are reduced per wave by transposing the per-point
+// hidden gradients through LDS (lane = hidden unit afterwards) and
+// accumulating in registers across the persistent loop; each wave writes one
+// partial row into a workspace that a second kernel reduces deterministically.
+#pragma once
+#include "ff_common.h"
+#include "ff_field_launch.h"
+
+namespace ff {
+
+// basis_dims [4,4,4,2,2,2]: the layout of every BASELINE workload but the
+// dictionary image set.
+struct Spec18 {
+  static constexpr int L = 6;
+  static constexpr int C = 18;
+  static constexpr int dim(int l) { return l < 3 ? 4 : 2; }
+  static constexpr int off(int l) { return l < 3 ? 4 * l : 12 + 2 * (l - 3); }
+};
+
+constexpr int kMatMode[3][2] = {{0, 1}, {0, 2}, {1, 2}};  // models/FactorFields.py:213
+constexpr int kVecMode[3] = {2, 1, 0};                     // models/FactorFields.py:214
+
+template <int D, int CK>
+struct XDim {
+  static constexpr int value = (CK == CK_GRID3 || CK == CK_VM) ? 3 : D;
+};
+
+// ---------------------------------------------------------------- coding
+// Basis level l at point x: bas[o .. o+dl) (models/FactorFields.py:487-490).
+template <int D, int dl, bool BF>
+__device__ __forceinline__ void basis_level_fwd(const ff_field_desc& fd, int l, const float* x,
+                                                float* out) {
+  const int R = fd.basis_reso[l];
+  float pos[D];
+  int Rs[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    pos[a] = unnorm_ac(map_coord(fd.mapping, x[a], fd.map_origin[a], fd.map_scale[l]), R);
+    Rs[a] = R;
+  }
+#pragma unroll
+  for (int c = 0; c < dl; ++c) out[c] = 0.f;
+  if (fd.basis_interp == FF_INTERP_BILINEAR) {
+    Corners<D> cr;
+    linear_corners<D>(pos, Rs, cr);
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k) {
+      float v[dl];
+      load_ch<dl, BF>(fd.basis[l], (int64_t)cr.tex[k] * dl, v);
+#pragma unroll
+      for (int c = 0; c < dl; ++c) out[c] = fmaf(v[c], cr.w[k], out[c]);
+    }
+  } else {
+    const int t = nearest_texel<D>(pos, Rs);
+    if (t >= 0) load_ch<dl, BF>(fd.basis[l], (int64_t)t * dl, out);
+  }
+}
+
+template <int D, int dl>
+__device__ __forceinline__ void basis_level_bwd(const ff_field_desc& fd, const ff_grads& g, int l,
+                                                const float* x, const float* grad) {
+  const int R = fd.basis_reso[l];
+  float pos[D];
+  int Rs[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    pos[a] = unnorm_ac(map_coord(fd.mapping, x[a], fd.map_origin[a], fd.map_scale[l]), R);
+    Rs[a] = R;
+  }
+  float* dst = g.basis[l];
+  if (fd.basis_interp == FF_INTERP_BILINEAR) {
+    Corners<D> cr;
+    linear_corners<D>(pos, Rs, cr);
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k) {
+      if (cr.w[k] != 0.f) {
+        float* p = dst + (int64_t)cr.tex[k] * dl;
+#pragma unroll
+        for (int c = 0; c < dl; ++c) unsafeAtomicAdd(p + c, cr.w[k] * grad[c]);
+      }
+    }
+  } else {
+    const int t = nearest_texel<D>(pos, Rs);
+    if (t >= 0) {
+      float* p = dst + (int64_t)t * dl;
+#pragma unroll
+      for (int c = 0; c < dl; ++c) unsafeAtomicAdd(p + c, grad[c]);
+    }
+  }
+}
+
+// Normalised coefficient coordinate (models/FactorFields.py:635-637).
+template <int XD>
+__device__ __forceinline__ void coeff_pts(const ff_field_desc& fd, const float* x, float* pts) {
+#pragma unroll
+  for (int a = 0; a < XD; ++a) pts[a] = (x[a] - fd.coeff_origin[a]) * fd.coeff_inv[a] - 1.f;
+}
+
+// Sample an ND-dim C-channel table at normalised pts (align_corners=False, border).
+template <int ND, int C, bool BF>
+__device__ __forceinline__ void grid_sample_border(const void* tab, const int* R, const float* pts,
+                                                   int interp, float* out) {
+  float pos[ND];
+#pragma unroll
+  for (int a = 0; a < ND; ++a) pos[a] = unnorm_border(pts[a], R[a]);
+#pragma unroll
+  for (int c = 0; c < C; ++c) out[c] = 0.f;
+  if (interp == FF_INTERP_BILINEAR) {
+    Corners<ND> cr;
+    linear_corners<ND>(pos, R, cr);
+#pragma unroll
+    for (int k = 0; k < (1 << ND); ++k) {
+      if (cr.w[k] != 0.f) {
+        float v[C];
+        load_ch<C, BF>(tab, (int64_t)cr.tex[k] * C, v);
+#pragma unroll
+        for (int c = 0; c < C; ++c) out[c] = fmaf(v[c], cr.w[k], out[c]);
+      }
+    }
+  } else {
+    const int t = nearest_texel<ND>(pos, R);
+    if (t >= 0) load_ch<C, BF>(tab, (int64_t)t * C, out);
+  }
+}
+
+template <int ND, int C>
+__device__ __forceinline__ void grid_scatter_border(float* dst, const int* R, const float* pts,
+                                                    int interp, const float* grad) {
+  float pos[ND];
+#pragma unroll
+  for (int a = 0; a < ND; ++a) pos[a] = unnorm_border(pts[a], R[a]);
+  if (interp == FF_INTERP_BILINEAR) {
+    Corners<ND> cr;
+    linear_corners<ND>(pos, R, cr);
+#pragma unroll
+    for (int k = 0; k < (1 << ND); ++k) {
+      if (cr.w[k] != 0.f) {
+        float* p = dst + (int64_t)cr.tex[k] * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) unsafeAtomicAdd(p + c, cr.w[k] * grad[c]);
+      }
+    }
+  } else {
+    const int t = nearest_texel<ND>(pos, R);
+    if (t >= 0) {
+      float* p = dst + (int64_t)t * C;
+#pragma unroll
+      for (int c = 0; c < C; ++c) unsafeAtomicAdd(p + c, grad[c]);
+    }
+  }
+}
+
+// VM plane m and line m values at pts (3-D).
+template <int C, bool BF>
+__device__ __forceinline__ void vm_terms(const ff_field_desc& fd, int m, const float* pts,
+                                         float* pv, float* lv) {
+  const int a0 = kMatMode[m][0], a1 = kMatMode[m][1], av = kVecMode[m];
+  const int Rp[2] = {fd.coeff_reso[a0], fd.coeff_reso[a1]};
+  const float pp[2] = {pts[a0], pts[a1]};
+  grid_sample_border<2, C, BF>(fd.coeff[m], Rp, pp, fd.coeff_interp, pv);
+  const int Rl[2] = {1, fd.coeff_reso[av]};
+  const float pl[2] = {0.f, pts[av]};
+  grid_sample_border<2, C, BF>(fd.coeff[3 + m], Rl, pl, fd.coeff_interp, lv);
+}
+
+// get_coding for one point: bas, cof (models/FactorFields.py:523-533).
+template <int D, int CK, bool BF, class S>
+__device__ __forceinline__ void coding_fwd(const ff_field_desc& fd, const float* x, float* bas,
+                                           float* cof) {
+  constexpr int C = S::C;
+  if (fd.basis_kind == FF_BASIS_GRID) {
+    static_for<0, S::L>([&](auto LI) {
+      constexpr int l = decltype(LI)::value;
+      basis_level_fwd<D, S::dim(l), BF>(fd, l, x, bas + S::off(l));
+    });
+  }
+  constexpr int XD = XDim<D, CK>::value;
+  if constexpr (CK == CK_GRID2 || CK == CK_GRID3) {
+    constexpr int ND = CK == CK_GRID2 ? 2 : 3;
+    float pts[XD];
+    coeff_pts<XD>(fd, x, pts);
+    grid_sample_border<ND, C, BF>(fd.coeff[0], fd.coeff_reso, pts, fd.coeff_interp, cof);
+  } else if constexpr (CK == CK_VM) {
+    float pts[3];
+    coeff_pts<3>(fd, x, pts);
+#pragma unroll 1
+    for (int m = 0; m < 3; ++m) {
+      float pv[C], lv[C];
+      vm_terms<C, BF>(fd, m, pts, pv, lv);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        float t = pv[c] * lv[c];
+        cof[c] = m == 0 ? t : cof[c] + t;
+      }
+    }
+  }
+}
+
+// Scatter d(bas) and d(cof) into the table gradients.
+template <int D, int CK, bool BF, class S>
+__device__ __forceinline__ void coding_bwd(const ff_field_desc& fd, const ff_grads& g,
+                                           const float* x, const float* dbas, const float* dcof) {
+  constexpr int C = S::C;
+  if (fd.basis_kind == FF_BASIS_GRID) {
+    static_for<0, S::L>([&](auto LI) {
+      constexpr int l = decltype(LI)::value;
+      basis_level_bwd<D, S::dim(l)>(fd, g, l, x, dbas + S::off(l));
+    });
+  }
+  constexpr int XD = XDim<D, CK>::value;
+  if constexpr (CK == CK_GRID2 || CK == CK_GRID3) {
+    constexpr int ND = CK == CK_GRID2 ? 2 : 3;
+    float pts[XD];
+    coeff_pts<XD>(fd, x, pts);
+    grid_scatter_border<ND, C>(g.coeff[0], fd.coeff_reso, pts, fd.coeff_interp, dcof);
+  } else if constexpr (CK == CK_VM) {
+    float pts[3];
+    coeff_pts<3>(fd, x, pts);
+#pragma unroll 1
+    for (int m = 0; m < 3; ++m) {
+      float pv[C], lv[C], t[C];
+      vm_terms<C, BF>(fd, m, pts, pv, lv);
+      const int a0 = kMatMode[m][0], a1 = kMatMode[m][1], av = kVecMode[m];
+      const int Rp[2] = {fd.coeff_reso[a0], fd.coeff_reso[a1]};
+      const float pp[2] = {pts[a0], pts[a1]};
+#pragma unroll
+      for (int c = 0; c < C; ++c) t[c] = dcof[c] * lv[c];
+      grid_scatter_border<2, C>(g.coeff[m], Rp, pp, fd.coeff_interp, t);
+      const int Rl[2] = {1, fd.coeff_reso[av]};
+      const float pl[2] = {0.f, pts[av]};
+#pragma unroll
+      for (int c = 0; c < C; ++c) t[c] = dcof[c] * pv[c];
+      grid_scatter_border<2, C>(g.coeff[3 + m], Rl, pl, fd.coeff_interp, t);
+    }
+  }
+}
+
+// feats = basis * coeff, with the 'none' variants of get_coding.
+template <int C, int CK>
+__device__ __forceinline__ void combine(const ff_field_desc& fd, const float* bas, const float* cof,
+                                       float* f) {
+  const bool hb = fd.basis_kind != FF_BASIS_NONE;
+  const bool hc = CK != CK_NONE;
+#pragma unroll
+  for (int c = 0; c < C; ++c) f[c] = (hb && hc) ? bas[c] * cof[c] : (hc ? cof[c] : bas[c]);
+}
+
+template <int C, int CK>
+__device__ __forceinline__ void combine_bwd(const ff_field_desc& fd, const float* bas,
+                                            const float* cof, const float* df, float* dbas,
+                                            float* dcof) {
+  const bool hb = fd.basis_kind != FF_BASIS_NONE;
+  const bool hc = CK != CK_NONE;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    dbas[c] = (hb && hc) ? df[c] * cof[c] : (hc ? 0.f : df[c]);
+    dcof[c] = (hb && hc) ? df[c] * bas[c] : (hc ? df[c] : 0.f);
+  }
+}
+
+// ---------------------------------------------------------------- decoder
+// Decoder weights are read through the constant address space so they are
+// fetched with scalar loads (uniform across the wave) even in kernels that
+// also issue atomics.
+typedef const __attribute__((address_space(4))) float* cfp;
+
+template <int IN, int H, int O, int NL>
+__device__ __forceinline__ void mlp_fwd(const ff_mlp_desc& m, const float* f, float* h, float* out) {
+  const cfp w0 = (cfp)m.w0;
+  if constexpr (NL == 1) {
+#pragma unroll
+    for (int o = 0; o < O; ++o) {
+      float a = 0.f;
+#pragma unroll
+      for (int i = 0; i < IN; ++i) a = fmaf(w0[o * IN + i], f[i], a);
+      out[o] = a;
+    }
+  } else {
+    const cfp b0 = (cfp)m.b0;
+    const cfp w1 = (cfp)m.w1;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      float a = b0[j];
+#pragma unroll
+      for (int i = 0; i < IN; ++i) a = fmaf(w0[j * IN + i], f[i], a);
+      h[j] = fmaxf(a, 0.f);
+    }
+#pragma unroll
+    for (int o = 0; o < O; ++o) {
+      float a = 0.f;
+#pragma unroll
+      for (int j = 0; j < H; ++j) a = fmaf(w1[o * H + j], h[j], a);
+      out[o] = a;
+    }
+  }
+}
+
+// ReLU pattern of the hidden layer as a bit mask (bit j = h[j] > 0).
+template <int H>
+__device__ __forceinline__ uint64_t relu_mask(const float* h) {
+  uint64_t msk = 0;
+#pragma unroll
+  for (int j = 0; j < H; ++j) msk |= (uint64_t)(h[j] > 0.f) << j;
+  return msk;
+}
+
+// d_out -> dh (NL == 2) and d_feats.
+template <int IN, int H, int O, int NL>
+__device__ __forceinline__ void mlp_bwd_local(const ff_mlp_desc& m, uint64_t msk, const float* dout,
+                                              float* dh, float* df) {
+  const cfp w0 = (cfp)m.w0;
+  if constexpr (NL == 1) {
+#pragma unroll
+    for (int i = 0; i < IN; ++i) {
+      float a = 0.f;
+#pragma unroll
+      for (int o = 0; o < O; ++o) a = fmaf(w0[o * IN + i], dout[o], a);
+      df[i] = a;
+    }
+  } else {
+    const cfp w1 = (cfp)m.w1;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      float a = 0.f;
+#pragma unroll
+      for (int o = 0; o < O; ++o) a = fmaf(w1[o * H + j], dout[o], a);
+      dh[j] = ((msk >> j) & 1) ? a : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < IN; ++i) {
+      float a = 0.f;
+#pragma unroll
+      for (int j = 0; j < H; ++j) a = fmaf(w0[j * IN + i], dh[j], a);
+      df[i] = a;
+    }
+  }
+}
+
+// Per-wave decoder weight-gradient accumulators.
+//   NL == 2 (H == 64): lane j owns row j of dW0 (IN values), db0[j] and
+//   column j of dW1 (O values).
+//   NL == 1: every lane holds the wave total of dW0 (O*IN values).
+template <int IN, int H, int O, int NL>
+struct MlpAcc {
+  static constexpr int kRow = NL == 1 ? O * IN : IN + 1 + O;
+  float v[kRow];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int k = 0; k < kRow; ++k) v[k] = 0.f;
+  }
+  // Number of floats one wave contributes to the workspace row.
+  static constexpr int kPartial = NL == 1 ? O * IN : H * IN + H + O * H;
+};
+
+__device__ __forceinline__ float lane_bcast(float v, int p) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), p));
+}
+
+// dW1[o][j] += sum_p dout[p][o] * h[p][j]: h is transposed through the
+// per-wave 64 x 65 LDS buffer so lane j reads column j; dout[p][o] is a
+// lane broadcast.  Inactive lanes pass dout = 0.
+template <int IN, int H, int O, int NL>
+__device__ __forceinline__ void wave_acc_w1(MlpAcc<IN, H, O, NL>& acc, float* lds, const float* h,
+                                            const float* dout) {
+  static_assert(NL == 2 && H == kWave, "hidden width must equal the wave size");
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int j = 0; j < H; ++j) lds[lane * 65 + j] = h[j];
+  __syncthreads();
+#pragma unroll 2
+  for (int p = 0; p < kWave; ++p) {
+    const float hv = lds[p * 65 + lane];
+#pragma unroll
+    for (int o = 0; o < O; ++o) acc.v[IN + 1 + o] = fmaf(lane_bcast(dout[o], p), hv, acc.v[IN + 1 + o]);
+  }
+  __syncthreads();
+}
+
+// dW0[j][i] += sum_p dh[p][j] * f[p][i] and db0[j] += sum_p dh[p][j].
+template <int IN, int H, int O, int NL>
+__device__ __forceinline__ void wave_acc_w0(MlpAcc<IN, H, O, NL>& acc, float* lds, const float* f,
+                                            const float* dh) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int j = 0; j < H; ++j) lds[lane * 65 + j] = dh[j];
+  __syncthreads();
+#pragma unroll 2
+  for (int p = 0; p < kWave; ++p) {
+    const float d = lds[p * 65 + lane];
+#pragma unroll
+    for (int i = 0; i < IN; ++i) acc.v[i] = fmaf(d, lane_bcast(f[i], p), acc.v[i]);
+    acc.v[IN] += d;
+  }
+  __syncthreads();
+}
+
+// NL == 1: dW0[o][i] += sum over the wave of dout[o] * f[i].
+template <int IN, int H, int O, int NL>
+__device__ __forceinline__ void wave_acc_linear(MlpAcc<IN, H, O, NL>& acc, const float* f,
+                                                const float* dout) {
+#pragma unroll
+  for (int o = 0; o < O; ++o)
+#pragma unroll
+    for (int i = 0; i < IN; ++i) acc.v[o * IN + i] += wave_sum(dout[o] * f[i]);
+}
+
+template <int IN, int H, int O, int NL>
+__device__ __forceinline__ void mlp_acc_store(const MlpAcc<IN, H, O, NL>& acc, float* row) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if constexpr (NL == 1) {
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < O * IN; ++k) row[k] = acc.v[k];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < IN; ++i) row[lane * IN + i] = acc.v[i];
+    row[H * IN + lane] = acc.v[IN];
+#pragma unroll
+    for (int o = 0; o < O; ++o) row[H * IN + H + o * H + lane] = acc.v[IN + 1 + o];
+  }
+}
+
+// Decoder forward + loss/d_out + decoder backward for one wave-tile; leaves
+// d_feats in df and accumulates the weight gradients.  mode 0: d_out read
+// from dsrc; 1: MSE against dsrc; 2: L1 against dsrc.
+template <int IN, int H, int O, int NL>
+__device__ __forceinline__ void mlp_train_tile(const ff_mlp_desc& m, MlpAcc<IN, H, O, NL>& acc,
+                                               float* lds, const float* f, bool active, int64_t pc,
+                                               const float* __restrict__ dsrc, int mode,
+                                               float grad_scale, float& lsum, float* df) {
+  float h[NL == 2 ? H : 1], o[O], dout[O];
+  mlp_fwd<IN, H, O, NL>(m, f, h, o);
+#pragma unroll
+  for (int k = 0; k < O; ++k) {
+    float d;
+    if (mode == 0) {
+      d = dsrc[pc * O + k];
+    } else {
+      const float e = o[k] - dsrc[pc * O + k];
+      if (mode == 1) {
+        d = 2.f * e * grad_scale;
+        lsum += active ? e * e : 0.f;
+      } else {
+        d = (e > 0.f ? 1.f : (e < 0.f ? -1.f : 0.f)) * grad_scale;
+        lsum += active ? fabsf(e) : 0.f;
+      }
+    }
+    dout[k] = active ? d : 0.f;
+  }
+  if constexpr (NL == 2) {
+    const uint64_t msk = relu_mask<H>(h);
+    wave_acc_w1<IN, H, O, NL>(acc, lds, h, dout);
+    float dh[H];
+    mlp_bwd_local<IN, H, O, NL>(m, msk, dout, dh, df);
+    wave_acc_w0<IN, H, O, NL>(acc, lds, f, dh);
+  } else {
+    mlp_bwd_local<IN, H, O, NL>(m, 0, dout, nullptr, df);
+    wave_acc_linear<IN, H, O, NL>(acc, f, dout);
+  }
+}
+
+// ---------------------------------------------------------------- kernels
+template <int D, int CK, bool BF, class S>
+__global__ __launch_bounds__(256) void coding_fwd_kernel(ff_field_desc fd, const float* __restrict__ x,
+                                                         int64_t n, float* __restrict__ feats,
+                                                         float* __restrict__ coeff,
+                                                         float* __restrict__ basis) {
+  constexpr int C = S::C;
+  constexpr int XD = XDim<D, CK>::value;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float xv[XD];
+#pragma unroll
+  for (int a = 0; a < XD; ++a) xv[a] = x[p * XD + a];
+  float bas[C], cof[C], f[C];
+  coding_fwd<D, CK, BF, S>(fd, xv, bas, cof);
+  combine<C, CK>(fd, bas, cof, f);
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    if (feats) feats[p * C + c] = f[c];
+    if (coeff) coeff[p * C + c] = CK == CK_NONE ? bas[c] : cof[c];
+    if (basis) basis[p * C + c] = fd.basis_kind == FF_BASIS_NONE ? cof[c] : bas[c];
+  }
+}
+
+template <int D, int CK, bool BF, class S>
+__global__ __launch_bounds__(256) void coding_bwd_kernel(ff_field_desc fd, ff_grads g,
+                                                         const float* __restrict__ x, int64_t n,
+                                                         const float* __restrict__ dfeat,
+                                                         const float* __restrict__ dcoeff,
+                                                         const float* __restrict__ dbasis) {
+  constexpr int C = S::C;
+  constexpr int XD = XDim<D, CK>::value;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float xv[XD];
+#pragma unroll
+  for (int a = 0; a < XD; ++a) xv[a] = x[p * XD + a];
+  float bas[C], cof[C], df[C], db[C], dc[C];
+  coding_fwd<D, CK, BF, S>(fd, xv, bas, cof);
+#pragma unroll
+  for (int c = 0; c < C; ++c) df[c] = dfeat ? dfeat[p * C + c] : 0.f;
+  combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
+  // the returned coeff aliases basis when the coefficient is 'none' (and vice versa)
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const float gc = dcoeff ? dcoeff[p * C + c] : 0.f;
+    const float gb = dbasis ? dbasis[p * C + c] : 0.f;
+    if (CK == CK_NONE) db[c] += gc + gb;
+    else if (fd.basis_kind == FF_BASIS_NONE) dc[c] += gc + gb;
+    else { dc[c] += gc; db[c] += gb; }
+  }
+  coding_bwd<D, CK, BF, S>(fd, g, xv, db, dc);
+}
+
+// Fused get_coding + linear_mat forward.
+template <int D, int CK, bool BF, class S, int H, int O, int NL>
+__global__ __launch_bounds__(256) void field_fwd_kernel(ff_field_desc fd, ff_mlp_desc m,
+                                                        const float* __restrict__ x, int64_t n,
+                                                        float* __restrict__ out) {
+  constexpr int C = S::C;
+  constexpr int XD = XDim<D, CK>::value;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float xv[XD];
+#pragma unroll
+  for (int a = 0; a < XD; ++a) xv[a] = x[p * XD + a];
+  float bas[C], cof[C], f[C], h[H], o[O];
+  coding_fwd<D, CK, BF, S>(fd, xv, bas, cof);
+  combine<C, CK>(fd, bas, cof, f);
+  mlp_fwd<C, H, O, NL>(m, f, h, o);
+#pragma unroll
+  for (int k = 0; k < O; ++k) out[p * O + k] = o[k];
+}
+
+// Fused backward (mode 0: d_out given) or regression training pass
+// (mode 1: MSE, mode 2: L1 against `target`).  Persistent: one wave per block,
+// blocks stride over 64-point tiles.
+template <int D, int CK, bool BF, class S, int H, int O, int NL>
+__global__ __launch_bounds__(64) void field_bwd_kernel(ff_field_desc fd, ff_mlp_desc m, ff_grads g,
+                                                       const float* __restrict__ x, int64_t n,
+                                                       const float* __restrict__ dsrc, int mode,
+                                                       float loss_scale, float* __restrict__ ws,
+                                                       float* __restrict__ loss_sum) {
+  constexpr int C = S::C;
+  constexpr int XD = XDim<D, CK>::value;
+  using Acc = MlpAcc<C, H, O, NL>;
+  __shared__ float lds[NL == 2 ? kWave * 65 : 1];
+  Acc acc;
+  acc.zero();
+  float lsum = 0.f;
+  const int lane = threadIdx.x;
+  const int64_t ntiles = (n + kWave - 1) / kWave;
+  const float grad_scale = (float)(1.0 / ((double)n * O)) * loss_scale;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t p = tile * kWave + lane;
+    const bool active = p < n;
+    const int64_t pc = active ? p : n - 1;
+    float xv[XD];
+#pragma unroll
+    for (int a = 0; a < XD; ++a) xv[a] = x[pc * XD + a];
+    float bas[C], cof[C], f[C], df[C];
+    coding_fwd<D, CK, BF, S>(fd, xv, bas, cof);
+    combine<C, CK>(fd, bas, cof, f);
+    mlp_train_tile<C, H, O, NL>(m, acc, lds, f, active, pc, dsrc, mode, grad_scale, lsum, df);
+    if (active) {
+      float db[C], dc[C];
+      combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
+      coding_bwd<D, CK, BF, S>(fd, g, xv, db, dc);
+    }
+  }
+  mlp_acc_store<C, H, O, NL>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
+  if (mode != 0) {
+    lsum = wave_sum(lsum);
+    if (lane == 0 && loss_sum) unsafeAtomicAdd(loss_sum, lsum);
+  }
+}
+
+// Fast decoder kernels (lane = point) for standalone linear_mat.
+template <int IN, int H, int O, int NL>
+__global__ __launch_bounds__(256) void mlp_fwd_kernel(ff_mlp_desc m, const float* __restrict__ f,
+                                                      int64_t n, float* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float fv[IN], h[H], o[O];
+#pragma unroll
+  for (int i = 0; i < IN; ++i) fv[i] = f[p * IN + i];
+  mlp_fwd<IN, H, O, NL>(m, fv, h, o);
+#pragma unroll
+  for (int k = 0; k < O; ++k) out[p * O + k] = o[k];
+}
+
+template <int IN, int H, int O, int NL>
+__global__ __launch_bounds__(64) void mlp_bwd_kernel(ff_mlp_desc m, const float* __restrict__ f,
+                                                     int64_t n, const float* __restrict__ dsrc,
+                                                     float* __restrict__ dfeat,
+                                                     float* __restrict__ ws) {
+  using Acc = MlpAcc<IN, H, O, NL>;
+  __shared__ float lds[NL == 2 ? kWave * 65 : 1];
+  Acc acc;
+  acc.zero();
+  float lsum = 0.f;
+  const int lane = threadIdx.x;
+  const int64_t ntiles = (n + kWave - 1) / kWave;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t p = tile * kWave + lane;
+    const bool active = p < n;
+    const int64_t pc = active ? p : n - 1;
+    float fv[IN], df[IN];
+#pragma unroll
+    for (int i = 0; i < IN; ++i) fv[i] = f[pc * IN + i];
+    mlp_train_tile<IN, H, O, NL>(m, acc, lds, fv, active, pc, dsrc, 0, 1.f, lsum, df);
+    if (active && dfeat) {
+#pragma unroll
+      for (int i = 0; i < IN; ++i) dfeat[p * IN + i] = df[i];
+    }
+  }
+  mlp_acc_store<IN, H, O, NL>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
+}
+
+}  // namespace ff
+
+
+// ---------------------------------------------------------------- launchers
+namespace ff {
+
+inline int launch_status() { return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH; }
+inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+template <int D, int CK, bool BF>
+int launch_coding_fwd(const ff_field_desc& d, const float* x, int64_t n, float* f, float* c,
+                      float* b, hipStream_t s) {
+  hipLaunchKernelGGL((coding_fwd_kernel<D, CK, BF, Spec18>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                     s, d, x, n, f, c, b);
+  return launch_status();
+}
+
+template <int D, int CK, bool BF>
+int launch_coding_bwd(const ff_field_desc& d, const ff_grads& g, const float* x, int64_t n,
+                      const float* df, const float* dc, const float* db, hipStream_t s) {
+  hipLaunchKernelGGL((coding_bwd_kernel<D, CK, BF, Spec18>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                     s, d, g, x, n, df, dc, db);
+  return launch_status();
+}
+
+template <int D, int CK, bool BF, int H, int O, int NL>
+int launch_field_fwd(const ff_field_desc& d, const ff_mlp_desc& m, const float* x, int64_t n,
+                     float* out, hipStream_t s) {
+  hipLaunchKernelGGL((field_fwd_kernel<D, CK, BF, Spec18, H, O, NL>), dim3(blocks_for(n, 256)),
+                     dim3(256), 0, s, d, m, x, n, out);
+  return launch_status();
+}
+
+template <int D, int CK, bool BF, int H, int O, int NL>
+int launch_field_bwd(const ff_field_desc& d, const ff_mlp_desc& m, const ff_grads& g, const float* x,
+                     int64_t n, const float* dsrc, int mode, float loss_scale, float* ws,
+                     unsigned blocks, float* loss_sum, hipStream_t s) {
+  hipLaunchKernelGGL((field_bwd_kernel<D, CK, BF, Spec18, H, O, NL>), dim3(blocks), dim3(64), 0, s, d,
+                     m, g, x, n, dsrc, mode, loss_scale, ws, loss_sum);
+  return launch_status();
+}
+
+template <int H, int O, int NL>
+int launch_mlp_fwd(const ff_mlp_desc& m, const float* f, int64_t n, float* out, hipStream_t s) {
+  hipLaunchKernelGGL((mlp_fwd_kernel<18, H, O, NL>), dim3(blocks_for(n, 256)), dim3(256), 0, s, m, f,
+                     n, out);
+  return launch_status();
+}
+
+template <int H, int O, int NL>
+int launch_mlp_bwd(const ff_mlp_desc& m, const float* f, int64_t n, const float* dout, float* dfeat,
+                   float* ws, unsigned blocks, hipStream_t s) {
+  hipLaunchKernelGGL((mlp_bwd_kernel<18, H, O, NL>), dim3(blocks), dim3(64), 0, s, m, f, n, dout,
+                     dfeat, ws);
+  return launch_status();
+}
+
+}  // namespace ff

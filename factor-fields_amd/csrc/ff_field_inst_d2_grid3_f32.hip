@@ -1,0 +1,5 @@
+// Field kernels for in_dim=2, coefficient GRID3, FP32 tables.
+#define FF_INST_D 2
+#define FF_INST_CK CK_GRID3
+#define FF_INST_BF false
+#include "ff_field_inst.inc"

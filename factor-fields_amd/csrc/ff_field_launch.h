@@ -1,0 +1,45 @@
+// Launcher declarations for the field kernels.  Each fast-path combination is
+// explicitly instantiated in its own translation unit (ff_field_inst_*.hip)
+// so the library builds in parallel; the C ABI (ff_field_api.hip) only sees
+// these declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ffields.h"
+
+namespace ff {
+
+// Internal coefficient kinds (spatial dims resolved).
+enum { CK_NONE = 0, CK_GRID2 = 1, CK_GRID3 = 2, CK_VM = 3 };
+
+template <int D, int CK, bool BF>
+int launch_coding_fwd(const ff_field_desc& d, const float* x, int64_t n, float* f, float* c,
+                      float* b, hipStream_t s);
+template <int D, int CK, bool BF>
+int launch_coding_bwd(const ff_field_desc& d, const ff_grads& g, const float* x, int64_t n,
+                      const float* df, const float* dc, const float* db, hipStream_t s);
+template <int D, int CK, bool BF, int H, int O, int NL>
+int launch_field_fwd(const ff_field_desc& d, const ff_mlp_desc& m, const float* x, int64_t n,
+                     float* out, hipStream_t s);
+template <int D, int CK, bool BF, int H, int O, int NL>
+int launch_field_bwd(const ff_field_desc& d, const ff_mlp_desc& m, const ff_grads& g, const float* x,
+                     int64_t n, const float* dsrc, int mode, float loss_scale, float* ws,
+                     unsigned blocks, float* loss_sum, hipStream_t s);
+template <int H, int O, int NL>
+int launch_mlp_fwd(const ff_mlp_desc& m, const float* f, int64_t n, float* out, hipStream_t s);
+template <int H, int O, int NL>
+int launch_mlp_bwd(const ff_mlp_desc& m, const float* f, int64_t n, const float* dout, float* dfeat,
+                   float* ws, unsigned blocks, hipStream_t s);
+
+// Generic fallbacks (any level dims / decoder shape), ff_field_generic.hip.
+int launch_coding_generic(const ff_field_desc& d, const ff_grads& g, int ck, int bwd, const float* x,
+                          int64_t n, float* feats, float* coeff, float* basis, const float* dfeat,
+                          const float* dcoeff, const float* dbasis, hipStream_t s);
+int launch_mlp_generic_fwd(const ff_mlp_desc& m, const float* f, int64_t n, float* out,
+                           hipStream_t s);
+int launch_mlp_generic_bwd(const ff_mlp_desc& m, const ff_grads& g, const float* f, int64_t n,
+                           const float* dout, float* dfeat, hipStream_t s);
+
+}  // namespace ff

@@ -1,0 +1,132 @@
+"""ctypes binding of libffields.so (the C ABI declared in include/ffields.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+the library is missing or fails to load, every call raises.  Structures below
+mirror the C structs field for field.
+"""
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libffields.so")
+
+FF_MAX_LEVELS = 8
+
+FF_OK, FF_ERR_ARG, FF_ERR_UNSUPPORTED, FF_ERR_WORKSPACE, FF_ERR_LAUNCH = range(5)
+MAP = {"sawtooth": 0, "triangle": 1, "sinc": 2, "x": 3}
+INTERP = {"bilinear": 0, "nearest": 1}
+BASIS_NONE, BASIS_GRID = 0, 1
+COEFF_NONE, COEFF_GRID, COEFF_VM = 0, 1, 2
+F32, BF16 = 0, 1
+LOSS = {"mse": 0, "l1": 1}
+
+c_i32 = ctypes.c_int32
+c_f32 = ctypes.c_float
+c_vp = ctypes.c_void_p
+
+
+class FieldDesc(ctypes.Structure):
+    _fields_ = [
+        ("in_dim", c_i32), ("x_dim", c_i32), ("n_levels", c_i32), ("n_channels", c_i32),
+        ("mapping", c_i32), ("basis_kind", c_i32), ("basis_interp", c_i32),
+        ("coeff_kind", c_i32), ("coeff_interp", c_i32), ("coeff_ndim", c_i32),
+        ("table_dtype", c_i32),
+        ("basis_dims", c_i32 * FF_MAX_LEVELS), ("basis_reso", c_i32 * FF_MAX_LEVELS),
+        ("coeff_reso", c_i32 * 3),
+        ("map_origin", c_f32 * 3), ("map_scale", c_f32 * FF_MAX_LEVELS),
+        ("coeff_origin", c_f32 * 3), ("coeff_inv", c_f32 * 3),
+        ("basis", c_vp * FF_MAX_LEVELS), ("coeff", c_vp * 6),
+    ]
+
+
+class MlpDesc(ctypes.Structure):
+    _fields_ = [("n_layers", c_i32), ("in_dim", c_i32), ("hidden", c_i32), ("out_dim", c_i32),
+                ("w0", c_vp), ("b0", c_vp), ("w1", c_vp)]
+
+
+class Grads(ctypes.Structure):
+    _fields_ = [("basis", c_vp * FF_MAX_LEVELS), ("coeff", c_vp * 6),
+                ("w0", c_vp), ("b0", c_vp), ("w1", c_vp)]
+
+
+class AdamTensor(ctypes.Structure):
+    _fields_ = [("param", c_vp), ("grad", c_vp), ("exp_avg", c_vp), ("exp_avg_sq", c_vp),
+                ("shadow_bf16", c_vp), ("n", ctypes.c_int64), ("lr", c_f32)]
+
+
+_lib = None
+_load_error = None
+
+
+def _declare(lib):
+    P = ctypes.POINTER
+    I64 = ctypes.c_int64
+    sz = ctypes.c_size_t
+    sig = {
+        "ff_status_str": (ctypes.c_char_p, [ctypes.c_int]),
+        "ff_version": (ctypes.c_int, []),
+        "ff_get_coding": (ctypes.c_int, [P(FieldDesc), c_vp, I64, c_vp, c_vp, c_vp, c_vp]),
+        "ff_get_coding_bwd": (ctypes.c_int, [P(FieldDesc), c_vp, I64, c_vp, c_vp, c_vp, P(Grads), c_vp]),
+        "ff_linear_mat_fwd": (ctypes.c_int, [P(MlpDesc), c_vp, I64, c_vp, c_vp]),
+        "ff_linear_mat_bwd_workspace_bytes": (sz, [P(MlpDesc), I64]),
+        "ff_linear_mat_bwd": (ctypes.c_int, [P(MlpDesc), c_vp, I64, c_vp, c_vp, P(Grads), c_vp, sz, c_vp]),
+        "ff_field_fwd": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), c_vp, I64, c_vp, c_vp]),
+        "ff_field_fused_supported": (ctypes.c_int, [P(FieldDesc), P(MlpDesc)]),
+        "ff_field_bwd_workspace_bytes": (sz, [P(FieldDesc), P(MlpDesc), I64]),
+        "ff_field_bwd": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), c_vp, I64, c_vp, P(Grads), c_vp, sz,
+                                        c_vp]),
+        "ff_field_train_regression": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), c_vp, c_vp, I64,
+                                                     ctypes.c_int, c_f32, P(Grads), c_vp, c_vp, sz,
+                                                     c_vp]),
+        "ff_adam_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, I64, c_f32, c_f32, c_f32, c_f32,
+                                        I64, ctypes.c_int, c_vp]),
+        "ff_adam_multi": (ctypes.c_int, [P(AdamTensor), ctypes.c_int, c_f32, c_f32, c_f32, I64,
+                                         ctypes.c_int, c_vp]),
+        "ff_cast_bf16": (ctypes.c_int, [c_vp, c_vp, I64, c_vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def exported_symbols():
+    """Names the C ABI header declares (checked against the library in tests)."""
+    import re
+    hdr = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include")
+    names = []
+    for f in sorted(os.listdir(hdr)):
+        if f.endswith(".h"):
+            txt = open(os.path.join(hdr, f)).read()
+            names += re.findall(r"^\s*(?:const\s+)?[\w]+\*?\s+\**(ff_\w+)\s*\(", txt, re.M)
+    return names
+
+
+def lib():
+    """The loaded library; raises if it cannot be loaded (no fallback)."""
+    global _lib, _load_error
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libffields.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        l = ctypes.CDLL(LIB_PATH)
+        _declare(l)
+        _lib = l
+    return _lib
+
+
+def check(status, what=""):
+    if status != FF_OK:
+        msg = lib().ff_status_str(status).decode()
+        raise RuntimeError(f"ffields: {what} failed: {msg} (status {status})")
+
+
+def stream_of(t=None):
+    dev = t.device if t is not None else torch.device("cuda", torch.cuda.current_device())
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())

@@ -1,0 +1,160 @@
+"""The benchmark workloads: model configs and seeded synthetic data.
+
+Each ``*_cfg`` returns a cfg tree with the reference's keys
+(configs/defaults.yaml layout) describing one workload; the data generators
+are seeded from the reference's base seed 20211202 (train_per_scene.py:240),
+config k using 20211202 + 1000*k.
+"""
+
+import math
+
+import torch
+
+from .config import AttrDict
+
+BASE_SEED = 20211202
+FREQ = [2.0, 3.2, 4.4, 5.6, 6.8, 8.0]
+RESOS = [32, 51, 70, 89, 108, 128]
+
+
+def _cfg(mode, model, dataset, renderer=None, training=None):
+    m = dict(basis_dims=[4, 4, 4, 2, 2, 2], basis_resos=list(RESOS), freq_bands=list(FREQ),
+             coeff_reso=32, total_params=0, T_basis=0, T_coeff=0, coef_init=1.0,
+             coef_mode="bilinear", basis_mode="bilinear", basis_mapping="sawtooth",
+             coeff_type="grid", basis_type="grid", with_dropout=False, num_layers=2, hidden_dim=64,
+             out_dim=3, param_derivation="literal")
+    m.update(model)
+    r = dict(fea2denseAct="softplus", density_shift=-10, distance_scale=25.0, step_ratio=0.5,
+             max_samples=512, alphaMask_thres=0.02, rayMarch_weight_thres=1e-3, shadingMode="SH")
+    r.update(renderer or {})
+    t = dict(batch_size=4096, n_iters=2000, lr_small=0.001, lr_large=0.02, volume_resoInit=128,
+             volume_resoFinal=128, linear_mat=True, coeff=True, basis=True, renderModule=True)
+    t.update(training or {})
+    d = dict(is_unbound=False, scene_reso=1024)
+    d.update(dataset)
+    return AttrDict.wrap(dict(defaults=dict(mode=mode), model=m, dataset=d, renderer=r, training=t))
+
+
+def image_cfg(size=512):
+    """2D image regression: grid basis [4,4,4,2,2,2]@[32..128], 64x64x18
+    coefficient grid, sawtooth, MLP 18->64->3, Adam lr 0.02."""
+    return _cfg("image", dict(coeff_reso=64, out_dim=3, coef_init=1.0),
+                dict(aabb=[[0.0, 0.0], [float(size), float(size)]]),
+                training=dict(batch_size=65536, n_iters=2000, lr_small=0.02, lr_large=0.02))
+
+
+def sdf_cfg():
+    """3D SDF regression: 3-D grid basis [4,4,4,2,2,2]@[32..128] over [-1,1]^3,
+    VM coefficient (three 128^2 planes + three 128 lines, 18 ch), MLP 18->64->1,
+    L1 loss."""
+    return _cfg("sdf", dict(coeff_type="vm", coeff_reso=128, out_dim=1, coef_init=0.2),
+                dict(aabb=[[-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]]),
+                training=dict(batch_size=1 << 20, n_iters=2000, lr_small=0.002, lr_large=0.02))
+
+
+def nerf_cfg():
+    """Per-scene NeRF: 3-D grid basis [4,4,4,2,2,2]@[32..128], 32^3x18
+    coefficient grid over [-1.5,1.5]^3 (reference keys: total_params 10,747,166,
+    coeff_reso 32, scene_reso 1024), degree-2 SH colour head (out_dim 1 + 27)."""
+    return _cfg("reconstruction", dict(coeff_reso=32, total_params=10747166, out_dim=28,
+                                       param_derivation="reference"),
+                dict(aabb=[[-1.5, -1.5, -1.5], [1.5, 1.5, 1.5]], scene_reso=1024),
+                training=dict(batch_size=4096, n_iters=30000, lr_small=0.001, lr_large=0.02))
+
+
+# ---------------------------------------------------------------- 2D image
+def synth_image(size=512, seed=BASE_SEED, n_gauss=64, n_sin=32, device="cpu"):
+    """[size, size, 3] float32 in [0,1]: 64 random 2D Gaussians (sigma U[4,64]
+    px) + 32 random sinusoids (U[1,40] cycles/image), min-max normalised per
+    channel (scaled to `size` from the 512 px definition)."""
+    g = torch.Generator().manual_seed(seed)
+    s = size / 512.0
+    yy, xx = torch.meshgrid(torch.arange(size, dtype=torch.float64) + 0.5,
+                            torch.arange(size, dtype=torch.float64) + 0.5, indexing="ij")
+    img = torch.zeros((size, size, 3), dtype=torch.float64)
+    for _ in range(n_gauss):
+        c = torch.rand(2, generator=g, dtype=torch.float64) * size
+        sig = (4 + torch.rand(1, generator=g, dtype=torch.float64) * 60) * s
+        amp = torch.rand(3, generator=g, dtype=torch.float64)
+        r2 = (xx - c[0]) ** 2 + (yy - c[1]) ** 2
+        img += torch.exp(-r2 / (2 * sig ** 2))[..., None] * amp
+    for _ in range(n_sin):
+        f = 1 + torch.rand(1, generator=g, dtype=torch.float64) * 39
+        th = torch.rand(1, generator=g, dtype=torch.float64) * 2 * math.pi
+        ph = torch.rand(1, generator=g, dtype=torch.float64) * 2 * math.pi
+        amp = torch.rand(3, generator=g, dtype=torch.float64) / math.sqrt(n_sin)
+        u = (xx * torch.cos(th) + yy * torch.sin(th)) / size
+        img += torch.sin(2 * math.pi * f * u + ph)[..., None] * amp
+    lo = img.amin(dim=(0, 1), keepdim=True)
+    hi = img.amax(dim=(0, 1), keepdim=True)
+    img = (img - lo) / (hi - lo)
+    return img.float().to(device)
+
+
+def pixel_coords(h, w, device="cpu"):
+    """Pixel centres (x+0.5, y+0.5) in row-major order (dataLoader/image.py:42-43)."""
+    y, x = torch.meshgrid(torch.arange(h, device=device), torch.arange(w, device=device), indexing="ij")
+    return (torch.stack((x, y), -1).float() + 0.5).reshape(-1, 2)
+
+
+# ---------------------------------------------------------------- 3D SDF
+class SdfScene:
+    """Union of 32 spheres (r U[0.05,0.25]) and 16 axis-aligned boxes
+    (half-extents U[0.05,0.3]) with centres U[-1,1]^3; exact SDF."""
+
+    def __init__(self, seed=BASE_SEED + 1000, n_sph=32, n_box=16):
+        g = torch.Generator().manual_seed(seed)
+        d = torch.float64
+        self.sc = torch.rand((n_sph, 3), generator=g, dtype=d) * 2 - 1
+        self.sr = 0.05 + torch.rand(n_sph, generator=g, dtype=d) * 0.2
+        self.bc = torch.rand((n_box, 3), generator=g, dtype=d) * 2 - 1
+        self.bh = 0.05 + torch.rand((n_box, 3), generator=g, dtype=d) * 0.25
+        self.g = g
+
+    def sdf(self, p):
+        p = p.to(torch.float64)
+        s = (torch.cdist(p, self.sc.to(p.device)) - self.sr.to(p.device)).amin(-1)
+        q = (p[:, None, :] - self.bc.to(p.device)).abs() - self.bh.to(p.device)
+        b = q.clamp(min=0).norm(dim=-1) + q.amax(-1).clamp(max=0)
+        return torch.minimum(s, b.amin(-1))
+
+    def surface_points(self, n, chunk=1 << 18):
+        """Area-weighted surface samples of the primitives kept where they lie
+        on the union's surface (|SDF| < 1e-5), offset by N(0, 0.01)."""
+        g = self.g
+        area_s = 4 * math.pi * self.sr ** 2
+        hx, hy, hz = self.bh[:, 0], self.bh[:, 1], self.bh[:, 2]
+        area_b = 8 * (hx * hy + hy * hz + hx * hz)
+        areas = torch.cat([area_s, area_b])
+        out = []
+        got = 0
+        while got < n:
+            k = torch.multinomial(areas, chunk, replacement=True, generator=g)
+            pts = torch.empty((chunk, 3), dtype=torch.float64)
+            is_s = k < len(self.sr)
+            ks = k[is_s]
+            v = torch.randn((len(ks), 3), generator=g, dtype=torch.float64)
+            pts[is_s] = self.sc[ks] + self.sr[ks, None] * v / v.norm(dim=-1, keepdim=True)
+            kb = k[~is_s] - len(self.sr)
+            h = self.bh[kb]
+            faces = torch.stack([h[:, 1] * h[:, 2], h[:, 0] * h[:, 2], h[:, 0] * h[:, 1]], -1)
+            ax = torch.multinomial(faces, 1, generator=g)[:, 0]
+            u = (torch.rand((len(kb), 3), generator=g, dtype=torch.float64) * 2 - 1) * h
+            sign = torch.where(torch.rand(len(kb), generator=g) < 0.5, -1.0, 1.0).to(torch.float64)
+            u[torch.arange(len(kb)), ax] = sign * h[torch.arange(len(kb)), ax]
+            pts[~is_s] = self.bc[kb] + u
+            keep = self.sdf(pts).abs() < 1e-5
+            pts = pts[keep]
+            out.append(pts)
+            got += len(pts)
+        pts = torch.cat(out)[:n]
+        return pts + torch.randn(pts.shape, generator=g, dtype=torch.float64) * 0.01
+
+    def training_pool(self, n):
+        """n points (half uniform in [-1,1]^3, half near-surface) and their SDF."""
+        g = self.g
+        nu = n // 2
+        uni = torch.rand((nu, 3), generator=g, dtype=torch.float64) * 2 - 1
+        surf = self.surface_points(n - nu)
+        pts = torch.cat([uni, surf])
+        return pts.float(), self.sdf(pts).float()[:, None]

@@ -1,0 +1,197 @@
+/*
+ * ffields.h -- C ABI of the MI355X (gfx950) field-evaluation and
+ * volume-rendering library (libffields.so).
+ *
+ * The reference (autonomousvision/factor-fields) has no native layer: its hot
+ * path is a chain of ATen calls inside the Python module
+ * models/FactorFields.py.  Each entry point below replaces one of those
+ * Python-level operations; the comment on each names the reference function
+ * (file:line) whose semantics it implements.
+ *
+ * Conventions
+ *   - Plain C11 types only.  All array arguments are DEVICE pointers.
+ *   - Every entry point enqueues on `stream` (a hipStream_t passed as void*)
+ *     and never synchronises the host.  Argument errors are reported
+ *     synchronously as a nonzero status (see ff_status_str); launch errors as
+ *     FF_ERR_LAUNCH.
+ *   - The caller owns every buffer; the library keeps no allocation and no
+ *     global mutable state.  Workspaces are passed in; size them with the
+ *     matching *_workspace_bytes query.
+ *   - Tables are texel-major: a grid of spatial size (D,)H,W with C channels is
+ *     stored as [(D,)H,W,C] (the channels_last layout of the reference's
+ *     [1,C,(D,)H,W] parameter), so one corner is one contiguous C-vector.
+ *   - Gradient buffers are FP32 and are ACCUMULATED into (+=); zero them first.
+ */
+#ifndef FFIELDS_H
+#define FFIELDS_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FF_MAX_LEVELS 8
+
+enum ff_status {
+  FF_OK = 0,
+  FF_ERR_ARG = 1,        /* bad argument (null pointer, size, unsupported combination) */
+  FF_ERR_UNSUPPORTED = 2,/* configuration without a kernel instantiation */
+  FF_ERR_WORKSPACE = 3,  /* workspace missing or too small */
+  FF_ERR_LAUNCH = 4      /* hipGetLastError after the launch was not hipSuccess */
+};
+
+/* models/FactorFields.py:11-33 (grid_mapping) */
+enum ff_mapping { FF_MAP_SAWTOOTH = 0, FF_MAP_TRIANGLE = 1, FF_MAP_SINC = 2, FF_MAP_X = 3 };
+/* grid_sample mode: cfg.model.basis_mode / coef_mode */
+enum ff_interp { FF_INTERP_BILINEAR = 0, FF_INTERP_NEAREST = 1 };
+/* cfg.model.basis_type */
+enum ff_basis_kind { FF_BASIS_NONE = 0, FF_BASIS_GRID = 1 };
+/* cfg.model.coeff_type: GRID = models/FactorFields.py:432-434; VM = sum of
+ * plane (x) line products over the three axis pairs (TensoRF-style, the
+ * combination the reference cannot form with a grid basis) */
+enum ff_coeff_kind { FF_COEFF_NONE = 0, FF_COEFF_GRID = 1, FF_COEFF_VM = 2 };
+enum ff_dtype { FF_F32 = 0, FF_BF16 = 1 };
+
+/*
+ * Field description: everything get_coeff/get_basis need
+ * (models/FactorFields.py:425-516 with the shapes from setup_params :262-307).
+ */
+typedef struct ff_field_desc {
+  int32_t in_dim;        /* spatial dims of the basis grids (2 or 3) */
+  int32_t x_dim;         /* coordinates per point: in_dim, or in_dim+1 in 'images' mode */
+  int32_t n_levels;      /* L = len(basis_dims) */
+  int32_t n_channels;    /* C = sum(basis_dims) */
+  int32_t mapping;       /* enum ff_mapping */
+  int32_t basis_kind;    /* enum ff_basis_kind */
+  int32_t basis_interp;  /* enum ff_interp */
+  int32_t coeff_kind;    /* enum ff_coeff_kind */
+  int32_t coeff_interp;  /* enum ff_interp */
+  int32_t coeff_ndim;    /* spatial dims of the coefficient grid (2 or 3) */
+  int32_t table_dtype;   /* enum ff_dtype of basis/coeff tables (grads are always FP32) */
+  int32_t basis_dims[FF_MAX_LEVELS];
+  int32_t basis_reso[FF_MAX_LEVELS];
+  int32_t coeff_reso[3]; /* extents along x, y, z (W, H, D) */
+  float map_origin[3];   /* aabb[0] used by grid_mapping */
+  float map_scale[FF_MAX_LEVELS]; /* FP32 max(aabb[1]-aabb[0]) / freq_bands[l] */
+  float coeff_origin[3]; /* aabb[0] used by normalize_coord */
+  float coeff_inv[3];    /* FP32 2 / (aabb[1]-aabb[0]) */
+  const void* basis[FF_MAX_LEVELS]; /* level tables [R^in_dim, d_l] */
+  const void* coeff[6];  /* GRID: [0]; VM: planes xy,xz,yz in [0..2], lines z,y,x in [3..5] */
+} ff_field_desc;
+
+/* models/FactorFields.py:113-159 (MLPMixer = linear_mat), pe = 0.
+ * n_layers 1: out = W0 x.  n_layers 2: out = W1 relu(W0 x + b0).
+ * Weights in torch.nn.Linear layout [out, in], FP32. */
+typedef struct ff_mlp_desc {
+  int32_t n_layers;
+  int32_t in_dim;
+  int32_t hidden;
+  int32_t out_dim;
+  const float* w0;
+  const float* b0;
+  const float* w1;
+} ff_mlp_desc;
+
+/* FP32 gradient accumulators matching ff_field_desc / ff_mlp_desc. */
+typedef struct ff_grads {
+  float* basis[FF_MAX_LEVELS];
+  float* coeff[6];
+  float* w0;
+  float* b0;
+  float* w1;
+} ff_grads;
+
+const char* ff_status_str(int status);
+int ff_version(void);
+
+/* ---------------------------------------------------------------- field --
+ * ff_get_coding: models/FactorFields.py:523-533 (get_coding), which calls
+ * get_coeff :425-465 and get_basis :467-516.  x [n, x_dim].  Outputs (any may
+ * be NULL): feats [n, C] = basis * coeff, coeff [n, C], basis [n, C].
+ */
+int ff_get_coding(const ff_field_desc* desc, const float* x, int64_t n,
+                  float* feats, float* coeff, float* basis, void* stream);
+
+/* Backward of ff_get_coding w.r.t. the tables.  d_feats [n, C] (may be NULL),
+ * d_coeff [n, C] (gradient reaching the returned coeff; may be NULL),
+ * d_basis [n, C] (gradient reaching a separately returned basis; may be NULL).
+ * Interpolation weights are recomputed from x. */
+int ff_get_coding_bwd(const ff_field_desc* desc, const float* x, int64_t n,
+                      const float* d_feats, const float* d_coeff, const float* d_basis,
+                      const ff_grads* grads, void* stream);
+
+/* ---------------------------------------------------------- linear_mat --
+ * models/FactorFields.py:144-159 (MLPMixer.forward).  feats [n, in], out [n, out].
+ */
+int ff_linear_mat_fwd(const ff_mlp_desc* mlp, const float* feats, int64_t n, float* out,
+                      void* stream);
+size_t ff_linear_mat_bwd_workspace_bytes(const ff_mlp_desc* mlp, int64_t n);
+/* d_out [n, out] -> d_feats [n, in] (may be NULL) and accumulated weight grads. */
+int ff_linear_mat_bwd(const ff_mlp_desc* mlp, const float* feats, int64_t n, const float* d_out,
+                      float* d_feats, const ff_grads* grads, void* ws, size_t ws_bytes,
+                      void* stream);
+
+/* ---------------------------------------------------- fused field path --
+ * get_coding followed by linear_mat in one kernel (the feats never leave
+ * registers): the reference's `linear_mat(get_coding(x)[0])`
+ * (scripts/2D_regression.ipynb:190-192, scripts/sdf_regression.ipynb:369-370,
+ * models/FactorFields.py:874-875).  out [n, out_dim].
+ */
+int ff_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const float* x, int64_t n,
+                 float* out, void* stream);
+/* 1 if ff_field_fwd / ff_field_bwd / ff_field_train_regression have a kernel
+ * for this (field, decoder) pair, else 0 (callers then compose ff_get_coding
+ * and ff_linear_mat). */
+int ff_field_fused_supported(const ff_field_desc* desc, const ff_mlp_desc* mlp);
+size_t ff_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp, int64_t n);
+/* Backward of ff_field_fwd given d_out [n, out_dim]; recomputes the forward. */
+int ff_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const float* x, int64_t n,
+                 const float* d_out, const ff_grads* grads, void* ws, size_t ws_bytes,
+                 void* stream);
+
+/* Regression training pass: forward, loss and backward fused
+ * (scripts/sdf_regression.ipynb:369-379, scripts/2D_regression.ipynb:190-206).
+ * loss_kind 0 = mean squared error, 1 = mean absolute error, both averaged
+ * over n*out_dim and multiplied by loss_scale.  target [n, out_dim].
+ * loss_sum (FP32 scalar, device) accumulates sum(err^2) or sum|err| unscaled,
+ * may be NULL. */
+enum ff_loss { FF_LOSS_MSE = 0, FF_LOSS_L1 = 1 };
+int ff_field_train_regression(const ff_field_desc* desc, const ff_mlp_desc* mlp, const float* x,
+                              const float* target, int64_t n, int loss_kind, float loss_scale,
+                              const ff_grads* grads, float* loss_sum, void* ws, size_t ws_bytes,
+                              void* stream);
+
+/* ----------------------------------------------------------- optimiser --
+ * torch.optim.Adam step (train_per_scene.py:132, betas (0.9, 0.99), eps 1e-8,
+ * no weight decay) fused over one flat parameter group: param/grad/exp_avg/
+ * exp_avg_sq [n] FP32; step is the 1-based step count after increment.
+ * If shadow_bf16 is non-NULL the updated parameter is also written there as
+ * BF16 (round to nearest even).  If zero_grad != 0 the grad is cleared. */
+int ff_adam_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq,
+                 uint16_t* shadow_bf16, int64_t n, float lr, float beta1, float beta2, float eps,
+                 int64_t step, int zero_grad, void* stream);
+
+/* The same Adam step over several tensors (e.g. every parameter of
+ * get_optparam_groups, models/FactorFields.py:541-554) in one launch; each
+ * tensor carries its group's learning rate. */
+typedef struct ff_adam_tensor {
+  float* param;
+  float* grad;
+  float* exp_avg;
+  float* exp_avg_sq;
+  uint16_t* shadow_bf16; /* may be NULL */
+  int64_t n;
+  float lr;
+} ff_adam_tensor;
+int ff_adam_multi(const ff_adam_tensor* tensors, int count, float beta1, float beta2, float eps,
+                  int64_t step, int zero_grad, void* stream);
+
+/* FP32 -> BF16 (round to nearest even) copy, used to refresh BF16 table shadows. */
+int ff_cast_bf16(const float* src, uint16_t* dst, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FFIELDS_H */

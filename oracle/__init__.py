@@ -1,0 +1,50 @@
+"""CPU oracle for the Factor Fields field-evaluation and volume-rendering path.
+
+TEST INFRASTRUCTURE ONLY.  This package is a CPU restatement of the reference
+algorithm (autonomousvision/factor-fields, ``models/FactorFields.py``,
+``models/sh.py``, ``dataLoader/ray_utils.py``, ``utils.py``) written with the
+same ATen operations and the same operation order as the reference, so that it
+reproduces the reference's FP32 semantics on the same inputs.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it, and only as the checker or the timed CPU baseline -- never as part of
+the product path, which must fail loudly when its HIP library is missing.
+
+How each function is pinned (checked against something other than itself):
+
+* ``grid_mapping``      -- closed forms, periodicity / range invariants and the
+                           identity special case (tests/test_oracle_mapping.py).
+* ``dct_dict``          -- closed-form DCT rows, selected-row tables and unit
+                           row norms (tests/test_oracle_init.py).
+* ``setup_params``      -- the parameter counts the reference notebooks printed
+                           (1,429,376 for configs/image.yaml; 5,342,274 for
+                           configs/sdf.yaml) and hand arithmetic.
+* ``grid_sample`` uses  -- an independent numpy brute-force bi/trilinear and
+                           nearest interpolator (tests/test_oracle_gather.py),
+                           partition of unity, FP64 finite differences for the
+                           gradients.
+* VM coefficient        -- new semantics (the reference's ``vm`` coefficient
+                           cannot be combined with a grid basis); pinned by the
+                           same numpy brute force on planes and lines.
+* ``MLPMixer``          -- FP64 matrix products in numpy.
+* sampler / occupancy   -- closed forms with injected jitter; exact integer
+                           properties (monotone t, in-box masks).
+* ``raw2alpha``/composite -- telescoping identity sum(w) + T_last = prod
+                           (with the +1e-10 term), single-sample and sigma = 0
+                           special cases, FP64 finite differences.
+* ``eval_sh``           -- orthonormality of the degree-2 real SH basis by
+                           quadrature on the sphere.
+* ``sample_pdf``        -- uniform-weights special case (stratified lerp) and a
+                           pure-Python inverse-CDF loop on tiny inputs.
+"""
+
+from .field import (AttrDict, N_to_reso, N_to_vm_reso, OracleField, cal_n_samples,
+                    dct_dict, grid_mapping, make_cfg, setup_params)
+from .render import (OracleRenderer, alpha_grid_sample, composite, eval_sh,
+                     raw2alpha, sample_pdf, sample_point)
+
+__all__ = [
+    "AttrDict", "N_to_reso", "N_to_vm_reso", "OracleField", "cal_n_samples",
+    "dct_dict", "grid_mapping", "make_cfg", "setup_params", "OracleRenderer",
+    "alpha_grid_sample", "composite", "eval_sh", "raw2alpha", "sample_pdf",
+    "sample_point",
+]

@@ -1,0 +1,181 @@
+"""GPU parity of the field path (get_coding, linear_mat, fused field, fused
+regression step) against the CPU oracle on the same seeded inputs.
+
+Bars: forward within 1e-5 relative (plus an absolute floor of 1e-6 of the
+output's max magnitude), gradients within 1e-4 relative (floor 1e-5), as the
+FP32 form must meet against the reference's grid_sample semantics."""
+import sys
+
+import pytest
+import torch
+
+from helpers import assert_close, new_model, oracle_of, rand_points, small_cfg
+
+pytestmark = pytest.mark.gpu
+
+FWD_RTOL = 1e-5
+GRAD_RTOL = 1e-4
+
+CASES = {
+    # name: cfg kwargs
+    "2d_grid_bilinear": dict(mode="image", D=2, ctype="grid"),
+    "2d_grid_nearest": dict(mode="image", D=2, ctype="grid", cmode="nearest", bmode="nearest"),
+    "2d_triangle": dict(mode="image", D=2, ctype="grid", mapping="triangle"),
+    "3d_grid": dict(mode="sdf", D=3, ctype="grid"),
+    "3d_vm": dict(mode="sdf", D=3, ctype="vm"),
+    "3d_vm_nearest": dict(mode="sdf", D=3, ctype="vm", cmode="nearest"),
+    "3d_sinc": dict(mode="sdf", D=3, ctype="grid", mapping="sinc"),
+    "3d_x_zeros_padding": dict(mode="sdf", D=3, ctype="grid", mapping="x"),
+    "generic_dims_2d": dict(mode="image", D=2, ctype="grid", dims=(3, 5, 2), resos=(7, 9, 4)),
+    "generic_dims_vm": dict(mode="sdf", D=3, ctype="vm", dims=(2, 3), resos=(6, 8)),
+}
+
+
+def _case(name):
+    print(f"[case] {name}", file=sys.stderr, flush=True)
+    return small_cfg(**CASES[name])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_get_coding_forward(name):
+    m = new_model(_case(name))
+    of = oracle_of(m)
+    x = rand_points(m, 4099)
+    feats, coeff = m.get_coding(x.cuda())
+    basis = m.get_basis(x.cuda())
+    rf, rc = of.get_coding(x)
+    assert_close(feats, rf, FWD_RTOL, "feats")
+    assert_close(coeff, rc, FWD_RTOL, "coeff")
+    assert_close(basis, of.get_basis(x), FWD_RTOL, "basis")
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_get_coding_backward(name):
+    m = new_model(_case(name))
+    of = oracle_of(m).requires_grad_(True)
+    x = rand_points(m, 3001, seed=4)
+    g = torch.Generator().manual_seed(9)
+    r1 = torch.randn((3001, m._spec.C), generator=g)
+    r2 = torch.randn((3001, m._spec.C), generator=g)
+    feats, coeff = m.get_coding(x.cuda())
+    ((feats * r1.cuda()).sum() + (coeff * r2.cuda()).sum()).backward()
+    rf, rc = of.get_coding(x)
+    ((rf * r1).sum() + (rc * r2).sum()).backward()
+    tabs = list(m.basises) + list(m.coeffs)
+    otabs = of.basises + of.coeffs
+    for k, (t, o) in enumerate(zip(tabs, otabs)):
+        assert_close(t.grad.contiguous(), o.grad, GRAD_RTOL, f"table {k}", atol_frac=1e-5)
+
+
+@pytest.mark.parametrize("layers,out_dim", [(2, 3), (2, 1), (2, 28), (1, 1), (2, 5)])
+def test_linear_mat(layers, out_dim):
+    m = new_model(small_cfg(mode="sdf", D=3, out_dim=out_dim, layers=layers))
+    of = oracle_of(m).requires_grad_(True)
+    g = torch.Generator().manual_seed(2)
+    f = torch.randn((2050, 18), generator=g)
+    fc = f.cuda().requires_grad_(True)
+    out = m.linear_mat(fc)
+    fo = f.clone().requires_grad_(True)
+    ref = of.linear_mat(fo)
+    assert_close(out, ref, FWD_RTOL, "out")
+    r = torch.randn(out.shape, generator=g)
+    (out * r.cuda()).sum().backward()
+    (ref * r).sum().backward()
+    assert_close(fc.grad, fo.grad, GRAD_RTOL, "d_feats", atol_frac=1e-5)
+    for l, (w, b) in enumerate(of.mlp):
+        lin = m.linear_mat.backbone[l]
+        assert_close(lin.weight.grad, w.grad, GRAD_RTOL, f"W{l}", atol_frac=1e-5)
+        if b is not None:
+            assert_close(lin.bias.grad, b.grad, GRAD_RTOL, f"b{l}", atol_frac=1e-5)
+
+
+FUSED = ["2d_grid_bilinear", "3d_grid", "3d_vm", "2d_grid_nearest"]
+
+
+@pytest.mark.parametrize("name", FUSED)
+@pytest.mark.parametrize("out_dim,layers", [(3, 2), (1, 2), (28, 2), (1, 1)])
+def test_field_fused(name, out_dim, layers):
+    kw = dict(CASES[name], out_dim=out_dim, layers=layers)
+    print(f"[case] fused {name} {out_dim} {layers}", file=sys.stderr, flush=True)
+    m = new_model(small_cfg(**kw))
+    assert m.fused_supported()
+    of = oracle_of(m).requires_grad_(True)
+    x = rand_points(m, 5000, seed=7)
+    out = m.field(x.cuda())
+    ref = of.field(x)
+    assert_close(out, ref, FWD_RTOL, "out")
+    g = torch.Generator().manual_seed(3)
+    r = torch.randn(out.shape, generator=g)
+    (out * r.cuda()).sum().backward()
+    (ref * r).sum().backward()
+    for k, (t, o) in enumerate(zip(list(m.basises) + list(m.coeffs), of.basises + of.coeffs)):
+        assert_close(t.grad.contiguous(), o.grad, GRAD_RTOL, f"table {k}", atol_frac=1e-5)
+    for l, (w, b) in enumerate(of.mlp):
+        lin = m.linear_mat.backbone[l]
+        assert_close(lin.weight.grad, w.grad, GRAD_RTOL, f"W{l}", atol_frac=1e-5)
+        if b is not None:
+            assert_close(lin.bias.grad, b.grad, GRAD_RTOL, f"b{l}", atol_frac=1e-5)
+
+
+@pytest.mark.parametrize("loss", ["mse", "l1"])
+@pytest.mark.parametrize("name", ["2d_grid_bilinear", "3d_vm"])
+def test_regression_step(name, loss):
+    m = new_model(small_cfg(**dict(CASES[name], out_dim=3 if name.startswith("2d") else 1)))
+    of = oracle_of(m).requires_grad_(True)
+    n = 6000
+    x = rand_points(m, n, seed=8)
+    tgt = torch.randn((n, m.cfg.model.out_dim), generator=torch.Generator().manual_seed(5))
+    ls = m.regression_step(x.cuda(), tgt.cuda(), loss=loss, loss_scale=0.7)
+    ref = of.field(x)
+    e = ref - tgt
+    lr = (e ** 2).mean() if loss == "mse" else e.abs().mean()
+    (lr * 0.7).backward()
+    s = (e ** 2).sum() if loss == "mse" else e.abs().sum()
+    assert abs(ls.item() - s.item()) <= 1e-4 * abs(s.item())
+    for k, (t, o) in enumerate(zip(list(m.basises) + list(m.coeffs), of.basises + of.coeffs)):
+        assert_close(t.grad.contiguous(), o.grad, GRAD_RTOL, f"table {k}", atol_frac=1e-5)
+    for l, (w, b) in enumerate(of.mlp):
+        lin = m.linear_mat.backbone[l]
+        assert_close(lin.weight.grad, w.grad, GRAD_RTOL, f"W{l}", atol_frac=1e-5)
+
+
+def test_bf16_tables_forward():
+    """BF16 tables: same result as the oracle evaluated on the BF16-rounded
+    tables (FP32 arithmetic)."""
+    cfg = small_cfg(mode="sdf", D=3, ctype="grid")
+    m = new_model(cfg, dtype="bf16")
+    of = oracle_of(m, bf16_tables=True)
+    x = rand_points(m, 4000, seed=12)
+    feats, coeff = m.get_coding(x.cuda())
+    rf, rc = of.get_coding(x)
+    assert_close(feats, rf, FWD_RTOL, "feats")
+    assert_close(m.field(x.cuda()), of.field(x), FWD_RTOL, "field")
+
+
+def test_empty_batch():
+    m = new_model(small_cfg(mode="sdf", D=3, ctype="vm"))
+    x = torch.zeros((0, 3), device="cuda")
+    feats, coeff = m.get_coding(x)
+    assert feats.shape == (0, 18)
+    assert m.field(x).shape == (0, 3)
+
+
+def test_adam_matches_torch():
+    from ffields import FusedAdam
+    torch.manual_seed(0)
+    p1 = torch.nn.Parameter(torch.randn(10007, device="cuda"))
+    p2 = torch.nn.Parameter(torch.randn((1, 4, 9, 9), device="cuda").contiguous(memory_format=torch.channels_last))
+    q1 = torch.nn.Parameter(p1.detach().clone())
+    q2 = torch.nn.Parameter(p2.detach().clone())
+    opt = FusedAdam([{"params": [p1], "lr": 0.02}, {"params": [p2], "lr": 0.001}], betas=(0.9, 0.99))
+    ref = torch.optim.Adam([{"params": [q1], "lr": 0.02}, {"params": [q2], "lr": 0.001}],
+                           betas=(0.9, 0.99), foreach=False)
+    for it in range(5):
+        g1 = torch.randn(p1.shape, device="cuda")
+        g2 = torch.randn(p2.shape, device="cuda").contiguous(memory_format=torch.channels_last)
+        p1.grad, p2.grad = g1.clone(), g2.clone()
+        q1.grad, q2.grad = g1.clone(), g2.clone()
+        opt.step()
+        ref.step()
+    assert_close(p1, q1, 1e-6, "p1", atol_frac=1e-7)
+    assert_close(p2, q2, 1e-6, "p2", atol_frac=1e-7)
