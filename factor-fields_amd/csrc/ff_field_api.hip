@@ -213,6 +213,12 @@ int field_bwd_common(const ff_field_desc* desc, const ff_mlp_desc* mlp, const fl
 }
 
 }  // namespace
+
+int launch_reduce_partials(const float* ws, int rows, int NL, int IN, int H, int O, const ff_grads& g,
+                           hipStream_t s) {
+  return reduce_partials(ws, rows, NL, IN, H, O, &g, s);
+}
+
 }  // namespace ff
 
 using namespace ff;

@@ -434,33 +434,18 @@ __device__ __forceinline__ void mlp_acc_store(const MlpAcc<IN, H, O, NL>& acc, f
   }
 }
 
-// Decoder forward + loss/d_out + decoder backward for one wave-tile; leaves
-// d_feats in df and accumulates the weight gradients.  mode 0: d_out read
-// from dsrc; 1: MSE against dsrc; 2: L1 against dsrc.
-template <int IN, int H, int O, int NL>
-__device__ __forceinline__ void mlp_train_tile(const ff_mlp_desc& m, MlpAcc<IN, H, O, NL>& acc,
-                                               float* lds, const float* f, bool active, int64_t pc,
-                                               const float* __restrict__ dsrc, int mode,
-                                               float grad_scale, float& lsum, float* df) {
+// Decoder forward, d_out from `make_dout(out, dout)`, decoder backward for one
+// wave-tile; leaves d_feats in df and accumulates the weight gradients.
+// Inactive lanes contribute nothing (their d_out is forced to zero).
+template <int IN, int H, int O, int NL, class DoutFn>
+__device__ __forceinline__ void mlp_train_tile_fn(const ff_mlp_desc& m, MlpAcc<IN, H, O, NL>& acc,
+                                                  float* lds, const float* f, bool active,
+                                                  DoutFn&& make_dout, float* df) {
   float h[NL == 2 ? H : 1], o[O], dout[O];
   mlp_fwd<IN, H, O, NL>(m, f, h, o);
+  make_dout(o, dout);
 #pragma unroll
-  for (int k = 0; k < O; ++k) {
-    float d;
-    if (mode == 0) {
-      d = dsrc[pc * O + k];
-    } else {
-      const float e = o[k] - dsrc[pc * O + k];
-      if (mode == 1) {
-        d = 2.f * e * grad_scale;
-        lsum += active ? e * e : 0.f;
-      } else {
-        d = (e > 0.f ? 1.f : (e < 0.f ? -1.f : 0.f)) * grad_scale;
-        lsum += active ? fabsf(e) : 0.f;
-      }
-    }
-    dout[k] = active ? d : 0.f;
-  }
+  for (int k = 0; k < O; ++k) dout[k] = active ? dout[k] : 0.f;
   if constexpr (NL == 2) {
     const uint64_t msk = relu_mask<H>(h);
     wave_acc_w1<IN, H, O, NL>(acc, lds, h, dout);
@@ -471,6 +456,34 @@ __device__ __forceinline__ void mlp_train_tile(const ff_mlp_desc& m, MlpAcc<IN, 
     mlp_bwd_local<IN, H, O, NL>(m, 0, dout, nullptr, df);
     wave_acc_linear<IN, H, O, NL>(acc, f, dout);
   }
+}
+
+// mode 0: d_out read from dsrc; 1: MSE against dsrc; 2: L1 against dsrc.
+template <int IN, int H, int O, int NL>
+__device__ __forceinline__ void mlp_train_tile(const ff_mlp_desc& m, MlpAcc<IN, H, O, NL>& acc,
+                                               float* lds, const float* f, bool active, int64_t pc,
+                                               const float* __restrict__ dsrc, int mode,
+                                               float grad_scale, float& lsum, float* df) {
+  mlp_train_tile_fn<IN, H, O, NL>(
+      m, acc, lds, f, active,
+      [&](const float* o, float* dout) {
+#pragma unroll
+        for (int k = 0; k < O; ++k) {
+          if (mode == 0) {
+            dout[k] = dsrc[pc * O + k];
+          } else {
+            const float e = o[k] - dsrc[pc * O + k];
+            if (mode == 1) {
+              dout[k] = 2.f * e * grad_scale;
+              lsum += active ? e * e : 0.f;
+            } else {
+              dout[k] = (e > 0.f ? 1.f : (e < 0.f ? -1.f : 0.f)) * grad_scale;
+              lsum += active ? fabsf(e) : 0.f;
+            }
+          }
+        }
+      },
+      df);
 }
 
 // ---------------------------------------------------------------- kernels

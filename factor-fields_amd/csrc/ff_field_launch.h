@@ -33,6 +33,11 @@ template <int H, int O, int NL>
 int launch_mlp_bwd(const ff_mlp_desc& m, const float* f, int64_t n, const float* dout, float* dfeat,
                    float* ws, unsigned blocks, hipStream_t s);
 
+// Deterministic reduction of per-wave decoder weight-gradient partials into
+// the gradient buffers (ff_field_api.hip).
+int launch_reduce_partials(const float* ws, int rows, int NL, int IN, int H, int O, const ff_grads& g,
+                           hipStream_t s);
+
 // Generic fallbacks (any level dims / decoder shape), ff_field_generic.hip.
 int launch_coding_generic(const ff_field_desc& d, const ff_grads& g, int ck, int bwd, const float* x,
                           int64_t n, float* feats, float* coeff, float* basis, const float* dfeat,

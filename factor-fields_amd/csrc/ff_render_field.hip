@@ -1,0 +1,252 @@
+// NeRF field over a packed sample list: get_coding + linear_mat + density
+// activation + degree-2 SH colour head in one kernel, and its fused backward.
+//
+// Reference: FactorFields.forward models/FactorFields.py:874-885 (get_coding,
+// linear_mat, basis2density :639-643) with the colour head
+// sigmoid(eval_sh(2, feat[1:28].view(3, 9), viewdir)) (models/sh.py:34-64).
+// The sample count is read on the device (ff_samples.ray_off[n_rays]); the
+// grid is persistent and every wave strides over 64-sample tiles.
+#include "ff_field_impl.h"
+
+namespace ff {
+
+constexpr float kC0 = 0.28209479177387814f;
+constexpr float kC1 = 0.4886025119029199f;
+constexpr float kC2_0 = 1.0925484305920792f;
+constexpr float kC2_1 = -1.0925484305920792f;
+constexpr float kC2_2 = 0.31539156525252005f;
+constexpr float kC2_3 = -1.0925484305920792f;
+constexpr float kC2_4 = 0.5462742152960396f;
+
+// eval_sh(2, sh, dir) with the reference's grouping; sh = o[1 + 9c .. 9c + 9].
+__device__ __forceinline__ float sh2_eval(const float* sh, float x, float y, float z) {
+  float r = kC0 * sh[0];
+  r = r - kC1 * y * sh[1];
+  r = r + kC1 * z * sh[2];
+  r = r - kC1 * x * sh[3];
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float xy = x * y, yz = y * z, xz = x * z;
+  r = r + kC2_0 * xy * sh[4];
+  r = r + kC2_1 * yz * sh[5];
+  r = r + kC2_2 * (2.f * zz - xx - yy) * sh[6];
+  r = r + kC2_3 * xz * sh[7];
+  r = r + kC2_4 * (xx - yy) * sh[8];
+  return r;
+}
+
+// d(eval_sh)/d(sh_k) = Y_k(dir).
+__device__ __forceinline__ void sh2_basis(float x, float y, float z, float* Y) {
+  const float xx = x * x, yy = y * y, zz = z * z;
+  Y[0] = kC0;
+  Y[1] = -(kC1 * y);
+  Y[2] = kC1 * z;
+  Y[3] = -(kC1 * x);
+  Y[4] = kC2_0 * (x * y);
+  Y[5] = kC2_1 * (y * z);
+  Y[6] = kC2_2 * (2.f * zz - xx - yy);
+  Y[7] = kC2_3 * (x * z);
+  Y[8] = kC2_4 * (xx - yy);
+}
+
+__device__ __forceinline__ float sigmoidf(float x) { return 1.f / (1.f + expf(-x)); }
+
+// torch.nn.functional.softplus (beta 1, threshold 20) and its derivative.
+__device__ __forceinline__ float softplusf(float x) { return x > 20.f ? x : log1pf(expf(x)); }
+__device__ __forceinline__ float softplus_grad(float x) {
+  if (x > 20.f) return 1.f;
+  const float z = expf(x);
+  return z / (z + 1.f);
+}
+
+template <int CK, bool BF>
+__global__ __launch_bounds__(256) void nerf_field_fwd_kernel(ff_field_desc fd, ff_mlp_desc m,
+                                                             ff_render_head hd,
+                                                             const float* __restrict__ rays,
+                                                             ff_samples s, int64_t n_rays,
+                                                             float* __restrict__ sigma,
+                                                             float* __restrict__ rgb) {
+  constexpr int C = Spec18::C;
+  const int64_t n = s.ray_off[n_rays];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float xv[3] = {s.xyz[i * 3 + 0], s.xyz[i * 3 + 1], s.xyz[i * 3 + 2]};
+    const int64_t rb = s.ray[i];
+    const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
+    float bas[C], cof[C], f[C], h[64], o[28];
+    coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
+    combine<C, CK>(fd, bas, cof, f);
+    mlp_fwd<C, 64, 28, 2>(m, f, h, o);
+    const float x0 = o[0] + hd.density_shift;
+    sigma[i] = hd.density_act == 0 ? softplusf(x0) : fmaxf(x0, 0.f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = sigmoidf(sh2_eval(o + 1 + 9 * c, dx, dy, dz));
+  }
+}
+
+template <int CK, bool BF>
+__global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff_mlp_desc m,
+                                                            ff_render_head hd, ff_grads g,
+                                                            const float* __restrict__ rays,
+                                                            ff_samples s, int64_t n_rays,
+                                                            const float* __restrict__ d_sample,
+                                                            float* __restrict__ ws) {
+  constexpr int C = Spec18::C;
+  using Acc = MlpAcc<C, 64, 28, 2>;
+  __shared__ float lds[kWave * 65];
+  Acc acc;
+  acc.zero();
+  const int lane = threadIdx.x;
+  const int64_t n = s.ray_off[n_rays];
+  const int64_t ntiles = (n + kWave - 1) / kWave;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t i = tile * kWave + lane;
+    const bool active = i < n;
+    const int64_t ic = active ? i : n - 1;
+    float xv[3] = {s.xyz[ic * 3 + 0], s.xyz[ic * 3 + 1], s.xyz[ic * 3 + 2]};
+    const int64_t rb = s.ray[ic];
+    const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
+    const float4 gs = reinterpret_cast<const float4*>(d_sample)[ic];
+    float bas[C], cof[C], f[C], df[C];
+    coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
+    combine<C, CK>(fd, bas, cof, f);
+    mlp_train_tile_fn<C, 64, 28, 2>(
+        m, acc, lds, f, active,
+        [&](const float* o, float* dout) {
+          const float x0 = o[0] + hd.density_shift;
+          dout[0] = gs.x * (hd.density_act == 0 ? softplus_grad(x0) : (x0 > 0.f ? 1.f : 0.f));
+          float Y[9];
+          sh2_basis(dx, dy, dz, Y);
+          const float gc[3] = {gs.y, gs.z, gs.w};
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const float y = sigmoidf(sh2_eval(o + 1 + 9 * c, dx, dy, dz));
+            const float dpre = gc[c] * (1.f - y) * y;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) dout[1 + 9 * c + k] = dpre * Y[k];
+          }
+        },
+        df);
+    if (active) {
+      float db[C], dc[C];
+      combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
+      coding_bwd<3, CK, BF, Spec18>(fd, g, xv, db, dc);
+    }
+  }
+  mlp_acc_store<C, 64, 28, 2>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
+}
+
+namespace {
+
+int rf_status() { return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH; }
+
+int cu_count_rf() {
+  int dev = 0, cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
+    return 256;
+  return cu;
+}
+
+bool nerf_supported(const ff_field_desc& d, const ff_mlp_desc& m) {
+  static const int dims[6] = {4, 4, 4, 2, 2, 2};
+  if (d.in_dim != 3 || d.x_dim != 3 || d.n_channels != 18 || d.basis_kind != FF_BASIS_GRID) return false;
+  if (d.n_levels != 6) return false;
+  for (int l = 0; l < 6; ++l)
+    if (d.basis_dims[l] != dims[l]) return false;
+  if (!(d.coeff_kind == FF_COEFF_GRID && d.coeff_ndim == 3) && d.coeff_kind != FF_COEFF_VM) return false;
+  if (d.coeff_kind == FF_COEFF_VM && d.table_dtype != FF_F32) return false;
+  if (m.n_layers != 2 || m.in_dim != 18 || m.hidden != 64 || m.out_dim != 28) return false;
+  if (!m.w0 || !m.b0 || !m.w1) return false;
+  for (int l = 0; l < 6; ++l)
+    if (!d.basis[l] || d.basis_reso[l] < 2) return false;
+  const int nc = d.coeff_kind == FF_COEFF_VM ? 6 : 1;
+  for (int k = 0; k < nc; ++k)
+    if (!d.coeff[k]) return false;
+  return true;
+}
+
+template <class F>
+int with_nerf(const ff_field_desc& d, F&& f) {
+  if (d.coeff_kind == FF_COEFF_VM) return f(std::integral_constant<int, CK_VM>{}, std::false_type{});
+  if (d.table_dtype == FF_BF16) return f(std::integral_constant<int, CK_GRID3>{}, std::true_type{});
+  return f(std::integral_constant<int, CK_GRID3>{}, std::false_type{});
+}
+
+unsigned rf_blocks(int64_t cap, int per, int64_t per_cu) {
+  const int64_t need = (cap + per - 1) / per;
+  const int64_t lim = (int64_t)cu_count_rf() * per_cu;
+  return (unsigned)(need < lim ? (need > 0 ? need : 1) : lim);
+}
+
+constexpr int kNerfPartial = 64 * 18 + 64 + 28 * 64;
+
+}  // namespace
+}  // namespace ff
+
+using namespace ff;
+
+extern "C" {
+
+int ff_render_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
+                        const float* rays, int64_t n_rays, const ff_samples* s, float* sigma,
+                        float* rgb, void* stream) {
+  if (!desc || !mlp || !head || !s || !s->ray_off || !s->xyz || !s->ray || !rays || !sigma || !rgb)
+    return FF_ERR_ARG;
+  if (n_rays < 0) return FF_ERR_ARG;
+  if (!nerf_supported(*desc, *mlp)) return FF_ERR_UNSUPPORTED;
+  if (s->cap <= 0 || n_rays == 0) return FF_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const ff_field_desc d = *desc;
+  const ff_mlp_desc m = *mlp;
+  const ff_render_head h = *head;
+  const ff_samples sv = *s;
+  const unsigned blocks = rf_blocks(s->cap, 256, 8);
+  return with_nerf(d, [&](auto CKi, auto BFi) {
+    constexpr int CK = decltype(CKi)::value;
+    constexpr bool BF = decltype(BFi)::value;
+    hipLaunchKernelGGL((nerf_field_fwd_kernel<CK, BF>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
+                       n_rays, sigma, rgb);
+    return rf_status();
+  });
+}
+
+size_t ff_render_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp,
+                                           int64_t cap) {
+  if (!desc || !mlp || cap <= 0 || !nerf_supported(*desc, *mlp)) return 0;
+  return (size_t)rf_blocks(cap, 64, 16) * kNerfPartial * 4;
+}
+
+int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
+                        const float* rays, int64_t n_rays, const ff_samples* s, const float* d_sample,
+                        const ff_grads* grads, void* ws, size_t ws_bytes, void* stream) {
+  if (!desc || !mlp || !head || !s || !s->ray_off || !s->xyz || !s->ray || !rays || !d_sample || !grads)
+    return FF_ERR_ARG;
+  if (n_rays < 0) return FF_ERR_ARG;
+  if (!nerf_supported(*desc, *mlp)) return FF_ERR_UNSUPPORTED;
+  for (int l = 0; l < 6; ++l)
+    if (!grads->basis[l]) return FF_ERR_ARG;
+  const int nc = desc->coeff_kind == FF_COEFF_VM ? 6 : 1;
+  for (int k = 0; k < nc; ++k)
+    if (!grads->coeff[k]) return FF_ERR_ARG;
+  if (!grads->w0 || !grads->b0 || !grads->w1) return FF_ERR_ARG;
+  if (s->cap <= 0 || n_rays == 0) return FF_OK;
+  const unsigned blocks = rf_blocks(s->cap, 64, 16);
+  if (!ws || ws_bytes < (size_t)blocks * kNerfPartial * 4) return FF_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const ff_field_desc d = *desc;
+  const ff_mlp_desc m = *mlp;
+  const ff_render_head h = *head;
+  const ff_samples sv = *s;
+  const ff_grads g = *grads;
+  int r = with_nerf(d, [&](auto CKi, auto BFi) {
+    constexpr int CK = decltype(CKi)::value;
+    constexpr bool BF = decltype(BFi)::value;
+    hipLaunchKernelGGL((nerf_field_bwd_kernel<CK, BF>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays, sv,
+                       n_rays, d_sample, (float*)ws);
+    return rf_status();
+  });
+  if (r) return r;
+  return launch_reduce_partials((const float*)ws, (int)blocks, 2, 18, 64, 28, g, st);
+}
+
+}  // extern "C"

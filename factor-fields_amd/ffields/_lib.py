@@ -57,8 +57,22 @@ class AdamTensor(ctypes.Structure):
                 ("shadow_bf16", c_vp), ("n", ctypes.c_int64), ("lr", c_f32)]
 
 
+class RaySampler(ctypes.Structure):
+    _fields_ = [("aabb_min", c_f32 * 3), ("aabb_max", c_f32 * 3), ("step_size", c_f32),
+                ("n_samples", c_i32), ("distance_scale", c_f32), ("occ", c_vp),
+                ("occ_reso", c_i32 * 3), ("occ_origin", c_f32 * 3), ("occ_inv", c_f32 * 3)]
+
+
+class Samples(ctypes.Structure):
+    _fields_ = [("xyz", c_vp), ("z", c_vp), ("dist", c_vp), ("ray", c_vp), ("idx", c_vp),
+                ("ray_off", c_vp), ("cap", ctypes.c_int64)]
+
+
+class RenderHead(ctypes.Structure):
+    _fields_ = [("density_act", c_i32), ("density_shift", c_f32)]
+
+
 _lib = None
-_load_error = None
 
 
 def _declare(lib):
@@ -86,6 +100,16 @@ def _declare(lib):
         "ff_adam_multi": (ctypes.c_int, [P(AdamTensor), ctypes.c_int, c_f32, c_f32, c_f32, I64,
                                          ctypes.c_int, c_vp]),
         "ff_cast_bf16": (ctypes.c_int, [c_vp, c_vp, I64, c_vp]),
+        "ff_sample_rays": (ctypes.c_int, [P(RaySampler), c_vp, c_vp, I64, P(Samples), c_vp]),
+        "ff_render_field_fwd": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), P(RenderHead), c_vp, I64,
+                                               P(Samples), c_vp, c_vp, c_vp]),
+        "ff_render_field_bwd_workspace_bytes": (sz, [P(FieldDesc), P(MlpDesc), I64]),
+        "ff_render_field_bwd": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), P(RenderHead), c_vp, I64,
+                                               P(Samples), c_vp, P(Grads), c_vp, sz, c_vp]),
+        "ff_composite_fwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,
+                                            c_vp, c_vp, c_vp, c_vp, c_vp]),
+        "ff_composite_bwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,
+                                            c_vp, c_vp, c_f32, c_vp, c_vp, c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

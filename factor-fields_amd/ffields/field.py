@@ -24,6 +24,7 @@ from . import _lib as L
 from .config import AttrDict
 from .ops import Field, FieldSpec, GetCoding, LinearMat, is_texel_major, mlp_desc, texel_major
 from .params import N_to_reso, dct_basis, setup_params, vm_reso
+from .render import AlphaGridMask, RenderMixin
 
 MAT_MODE = [[0, 1], [0, 2], [1, 2]]   # models/FactorFields.py:213
 VEC_MODE = [2, 1, 0]                  # models/FactorFields.py:214
@@ -62,7 +63,7 @@ class MLPMixer(nn.Module):
         return LinearMat.apply(x, *self.params_list())
 
 
-class FactorFields(nn.Module):
+class FactorFields(RenderMixin, nn.Module):
     def __init__(self, cfg, device="cuda", table_dtype="fp32"):
         super().__init__()
         cfg = AttrDict.wrap(cfg)
@@ -95,8 +96,7 @@ class FactorFields(nn.Module):
             self.refresh_shadows()
         self._build_spec()
         if "reconstruction" in cfg.defaults.mode:
-            from .render import attach_render_state
-            attach_render_state(self)
+            self._init_render()
 
     # ------------------------------------------------ models/FactorFields.py:262-307
     def setup_params(self, aabb):
@@ -341,7 +341,6 @@ class FactorFields(nn.Module):
 
     def load(self, ckpt):
         if "alphaMask.aabb" in ckpt:
-            from .render import AlphaGridMask
             length = int(np.prod(ckpt["alphaMask.shape"]))
             vol = torch.from_numpy(np.unpackbits(ckpt["alphaMask.mask"])[:length].reshape(
                 ckpt["alphaMask.shape"]))

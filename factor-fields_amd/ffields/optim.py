@@ -9,6 +9,15 @@ import torch
 from . import _lib as L
 
 
+def _dense(t):
+    """True if t occupies one contiguous block in one of the layouts used here."""
+    if t.is_contiguous():
+        return True
+    if t.dim() == 4 and t.is_contiguous(memory_format=torch.channels_last):
+        return True
+    return t.dim() == 5 and t.is_contiguous(memory_format=torch.channels_last_3d)
+
+
 class FusedAdam(torch.optim.Optimizer):
     def __init__(self, params, lr=1e-3, betas=(0.9, 0.99), eps=1e-8, shadow_of=None):
         super().__init__(params, dict(lr=lr, betas=betas, eps=eps))
@@ -29,7 +38,7 @@ class FusedAdam(torch.optim.Optimizer):
                 g = p.grad
                 if p.dtype != torch.float32 or g.dtype != torch.float32:
                     raise TypeError("FusedAdam needs float32 parameters and gradients")
-                if g.stride() != p.stride() or not p.is_non_overlapping_and_dense():
+                if g.stride() != p.stride() or not _dense(p):
                     raise RuntimeError("gradient layout differs from its parameter")
                 st = self.state[p]
                 if not st:

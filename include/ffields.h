@@ -191,6 +191,82 @@ int ff_adam_multi(const ff_adam_tensor* tensors, int count, float beta1, float b
 /* FP32 -> BF16 (round to nearest even) copy, used to refresh BF16 table shadows. */
 int ff_cast_bf16(const float* src, uint16_t* dst, int64_t n, void* stream);
 
+
+/* ============================================================ rendering ==
+ * Ray sampling with occupancy skipping: models/FactorFields.py:586-602
+ * (sample_point), :91-110 (AlphaGridMask.sample_alpha) and the masking of
+ * FactorFields.forward :859-867.  Surviving samples are compacted ray-major
+ * into a packed list (the reference's boolean indexing xyz_sampled[ray_valid],
+ * :874); the sample count per ray and the candidate index of every sample
+ * reproduce the reference's masks exactly. */
+typedef struct ff_ray_sampler {
+  float aabb_min[3];     /* model.aabb[0] */
+  float aabb_max[3];     /* model.aabb[1] */
+  float step_size;       /* FP32 stepSize (update_renderParams :693-699) */
+  int32_t n_samples;     /* candidates per ray (N_samples / nSamples), <= 2048 */
+  float distance_scale;  /* cfg.renderer.distance_scale */
+  const uint8_t* occ;    /* occupancy volume [Dz, Hy, Wx] of 0/1, NULL = no AlphaGridMask */
+  int32_t occ_reso[3];   /* Wx, Hy, Dz */
+  float occ_origin[3];   /* alphaMask.aabb[0] */
+  float occ_inv[3];      /* FP32 1.0 / (aabb[1]-aabb[0]) * 2 (AlphaGridMask.invgridSize) */
+} ff_ray_sampler;
+
+/* Packed sample list (structure of arrays, capacity cap >= n_rays * n_samples). */
+typedef struct ff_samples {
+  float* xyz;            /* [cap, 3] sample positions */
+  float* z;              /* [cap] t along the ray (z_vals) */
+  float* dist;           /* [cap] (t[i+1]-t[i]) * distance_scale, 0 for the last candidate */
+  int32_t* ray;          /* [cap] ray index */
+  int32_t* idx;          /* [cap] candidate index along the ray */
+  int32_t* ray_off;      /* [n_rays + 1] exclusive offsets; ray_off[n_rays] = total */
+  int64_t cap;
+} ff_samples;
+
+/* rays [n_rays, 6] = (origin, direction); jitter [n_rays] U[0,1) per ray
+ * (the reference's torch.rand_like(rng[:, [0]]), :595) or NULL for
+ * is_train=False.  The total sample count stays on the device. */
+int ff_sample_rays(const ff_ray_sampler* sampler, const float* rays, const float* jitter,
+                   int64_t n_rays, const ff_samples* out, void* stream);
+
+/* Density / colour head of the NeRF field (models/FactorFields.py:639-643,
+ * basis2density; degree-2 spherical harmonics of models/sh.py:34-64 on
+ * decoder outputs 1..27 followed by a sigmoid, as MLPRender_Fea's output). */
+typedef struct ff_render_head {
+  int32_t density_act;   /* 0 softplus, 1 relu */
+  float density_shift;   /* cfg.renderer.density_shift */
+} ff_render_head;
+
+/* Field + head over the packed samples (count read on the device):
+ * sigma [cap], rgb [cap, 3].  rays supplies each sample's view direction. */
+int ff_render_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp,
+                        const ff_render_head* head, const float* rays, int64_t n_rays,
+                        const ff_samples* s, float* sigma, float* rgb, void* stream);
+size_t ff_render_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp,
+                                           int64_t cap);
+/* d_sample [cap, 4] = (dL/dsigma, dL/drgb[3]) per sample -> table and decoder grads. */
+int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp,
+                        const ff_render_head* head, const float* rays, int64_t n_rays,
+                        const ff_samples* s, const float* d_sample, const ff_grads* grads, void* ws,
+                        size_t ws_bytes, void* stream);
+
+/* Alpha compositing, one wavefront per ray with a multiplicative prefix scan
+ * for the transmittance: raw2alpha :82-88 and the composite of :878-896
+ * (app mask weight > weight_thres, white background, clamp).  Outputs:
+ * rgb_map [n_rays, 3] (clamped), rgb_raw [n_rays, 3] (before the clamp),
+ * depth [n_rays], acc [n_rays], weights [cap] (may be NULL). */
+int ff_composite_fwd(const ff_samples* s, int64_t n_rays, const float* sigma, const float* rgb,
+                     int white_bg, float weight_thres, float* rgb_map, float* rgb_raw, float* depth,
+                     float* acc, float* weights, void* stream);
+/* Backward of ff_composite_fwd by a reverse affine scan (no stored
+ * transmittance).  Either d_rgb_map [n_rays, 3] is given, or target
+ * [n_rays, 3] (then the loss is mean((rgb_map - target)^2) * loss_scale,
+ * train_per_scene.py:158, and loss_sum accumulates sum((rgb_map-target)^2)).
+ * Writes d_sample [cap, 4]. */
+int ff_composite_bwd(const ff_samples* s, int64_t n_rays, const float* sigma, const float* rgb,
+                     int white_bg, float weight_thres, const float* rgb_raw,
+                     const float* d_rgb_map, const float* target, float loss_scale,
+                     float* loss_sum, float* d_sample, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
