@@ -23,6 +23,8 @@ struct AdamBatch {
   uint16_t* shadow[kAdamMax];
   int64_t n[kAdamMax];
   float step_size[kAdamMax];  // lr / bias_correction1
+  int32_t reps[kAdamMax];
+  int64_t rstride[kAdamMax];
   int chunk_start[kAdamMax + 1];
   int count;
 };
@@ -40,11 +42,14 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamBatch b, float beta1, flo
   float* __restrict__ V = b.v[t];
   uint16_t* __restrict__ S = b.shadow[t];
   const float ss = b.step_size[t];
+  const int R = b.reps[t];
+  const int64_t rs = b.rstride[t];
   const float w1 = 1.f - beta1, w2 = 1.f - beta2;
   for (int k = threadIdx.x; k < kAdamChunk; k += blockDim.x) {
     const int64_t i = base + k;
     if (i >= n) break;
-    const float g = G[i];
+    float g = G[i];
+    for (int r = 1; r < R; ++r) g += G[i + r * rs];
     float m = M[i];
     m = m + w1 * (g - m);
     float v = V[i] * beta2;
@@ -55,8 +60,21 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamBatch b, float beta1, flo
     V[i] = v;
     P[i] = p;
     if (S) S[i] = f32_to_bf16(p);
-    if (zero_grad) G[i] = 0.f;
+    if (zero_grad)
+      for (int r = 0; r < R; ++r) G[i + r * rs] = 0.f;
   }
+}
+
+__global__ __launch_bounds__(256) void reduce_replicas_kernel(float* __restrict__ a, int R, int64_t n,
+                                                              int64_t stride) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = a[i];
+  for (int r = 1; r < R; ++r) {
+    s += a[i + r * stride];
+    a[i + r * stride] = 0.f;
+  }
+  a[i] = s;
 }
 
 __global__ __launch_bounds__(256) void cast_bf16_kernel(const float* __restrict__ src,
@@ -92,6 +110,9 @@ int ff_adam_multi(const ff_adam_tensor* tensors, int count, float beta1, float b
       b.param[k] = t.param; b.grad[k] = t.grad; b.m[k] = t.exp_avg; b.v[k] = t.exp_avg_sq;
       b.shadow[k] = t.shadow_bf16; b.n[k] = t.n;
       b.step_size[k] = (float)((double)t.lr / bc1);
+      b.reps[k] = t.grad_replicas > 1 ? t.grad_replicas : 1;
+      b.rstride[k] = t.grad_stride;
+      if (b.reps[k] > 1 && t.grad_stride < t.n) return FF_ERR_ARG;
       b.chunk_start[k] = chunks;
       chunks += (int)c;
       ++k;
@@ -112,8 +133,16 @@ int ff_adam_multi(const ff_adam_tensor* tensors, int count, float beta1, float b
 int ff_adam_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq, uint16_t* shadow_bf16,
                  int64_t n, float lr, float beta1, float beta2, float eps, int64_t step,
                  int zero_grad, void* stream) {
-  ff_adam_tensor t{param, grad, exp_avg, exp_avg_sq, shadow_bf16, n, lr};
+  ff_adam_tensor t{param, grad, exp_avg, exp_avg_sq, shadow_bf16, n, lr, 1, 0};
   return ff_adam_multi(&t, 1, beta1, beta2, eps, step, zero_grad, stream);
+}
+
+int ff_reduce_replicas(float* arena, int32_t replicas, int64_t n, int64_t stride, void* stream) {
+  if (!arena || replicas < 1 || n < 0 || (replicas > 1 && stride < n)) return FF_ERR_ARG;
+  if (replicas == 1 || n == 0) return FF_OK;
+  hipLaunchKernelGGL(reduce_replicas_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, arena, replicas, n, stride);
+  return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
 }
 
 int ff_cast_bf16(const float* src, uint16_t* dst, int64_t n, void* stream) {

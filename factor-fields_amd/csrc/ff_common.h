@@ -205,6 +205,24 @@ __device__ __forceinline__ int nearest_texel(const float* pos, const int* R) {
   return ok ? t : -1;
 }
 
+// ------------------------------------------------------------ XCD-local gradients
+// XCC_ID hardware register (which XCD this wave runs on).
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 0xf; }
+
+// Table-gradient pointers of the replica owned by this wave's XCD.
+__device__ __forceinline__ ff_grads local_grads(const ff_grads& g) {
+  if (g.replicas <= 1) return g;
+  ff_grads r = g;
+  const int64_t off = (int64_t)(xcc_id() % g.replicas) * g.replica_stride;
+#pragma unroll
+  for (int l = 0; l < FF_MAX_LEVELS; ++l)
+    if (r.basis[l]) r.basis[l] += off;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    if (r.coeff[k]) r.coeff[k] += off;
+  return r;
+}
+
 // ------------------------------------------------------------ wave reductions
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll

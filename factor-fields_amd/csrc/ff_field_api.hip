@@ -7,29 +7,38 @@
 
 namespace ff {
 
-__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ ws, int rows,
-                                                              int P, int n_w0, int n_b0,
-                                                              float* __restrict__ gw0,
-                                                              float* __restrict__ gb0,
-                                                              float* __restrict__ gw1) {
-  // Sum workspace rows [rows, P] into the decoder gradients (fixed order).
-  // Row layout (NL == 2): [dW0 (H*IN, torch row-major) | db0 (H) | dW1 (O*H)];
-  // (NL == 1): [dW0 (O*IN)].
+constexpr int kReduceSplit = 32;
+
+// Sum workspace rows [rows, P] into the decoder gradients in a fixed order:
+// stage 1 sums row slices r = y, y + 32, ... into part2[y][k]; stage 2 sums the
+// 32 slices.  Row layout (NL == 2): [dW0 (H*IN, torch row-major) | db0 (H) |
+// dW1 (O*H)]; (NL == 1): [dW0 (O*IN)].
+__global__ __launch_bounds__(256) void reduce_partials_stage1(const float* __restrict__ ws, int rows, int P,
+                                                              float* __restrict__ part2) {
   __shared__ float part[4][64];
   const int k = blockIdx.x * 64 + (threadIdx.x & 63);
   const int grp = threadIdx.x >> 6;
   float s = 0.f;
   if (k < P)
-    for (int r = grp; r < rows; r += 4) s += ws[(int64_t)r * P + k];
+    for (int r = blockIdx.y + kReduceSplit * grp; r < rows; r += 4 * kReduceSplit) s += ws[(int64_t)r * P + k];
   part[grp][threadIdx.x & 63] = s;
   __syncthreads();
-  if (grp == 0 && k < P) {
-    const float t = (part[0][threadIdx.x] + part[1][threadIdx.x]) +
-                    (part[2][threadIdx.x] + part[3][threadIdx.x]);
-    if (k < n_w0) gw0[k] += t;
-    else if (k < n_w0 + n_b0) gb0[k - n_w0] += t;
-    else gw1[k - n_w0 - n_b0] += t;
-  }
+  if (grp == 0 && k < P)
+    part2[(int64_t)blockIdx.y * P + k] = (part[0][threadIdx.x] + part[1][threadIdx.x]) +
+                                         (part[2][threadIdx.x] + part[3][threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void reduce_partials_stage2(const float* __restrict__ part2, int P, int n_w0,
+                                                              int n_b0, float* __restrict__ gw0,
+                                                              float* __restrict__ gb0,
+                                                              float* __restrict__ gw1) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  float t = 0.f;
+  for (int y = 0; y < kReduceSplit; ++y) t += part2[(int64_t)y * P + k];
+  if (k < n_w0) gw0[k] += t;
+  else if (k < n_w0 + n_b0) gb0[k - n_w0] += t;
+  else gw1[k - n_w0 - n_b0] += t;
 }
 
 namespace {
@@ -172,8 +181,11 @@ int reduce_partials(const float* ws, int rows, int NL, int IN, int H, int O, con
   const int P = partial_floats(NL, IN, H, O);
   const int n_w0 = NL == 1 ? O * IN : H * IN;
   const int n_b0 = NL == 1 ? 0 : H;
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3((P + 63) / 64), dim3(256), 0, s, ws, rows, P, n_w0,
-                     n_b0, g->w0, g->b0, g->w1);
+  float* part2 = const_cast<float*>(ws) + (int64_t)rows * P;   // workspace tail
+  hipLaunchKernelGGL(reduce_partials_stage1, dim3((P + 63) / 64, kReduceSplit), dim3(256), 0, s, ws, rows,
+                     P, part2);
+  hipLaunchKernelGGL(reduce_partials_stage2, dim3((P + 255) / 256), dim3(256), 0, s, part2, P, n_w0, n_b0,
+                     g->w0, g->b0, g->w1);
   return api_status();
 }
 
@@ -202,7 +214,7 @@ int field_bwd_common(const ff_field_desc* desc, const ff_mlp_desc* mlp, const fl
     return with_mlp(m, d.n_channels, [&](auto NLi, auto Hi, auto Oi) {
       constexpr int NL = decltype(NLi)::value, H = decltype(Hi)::value, O = decltype(Oi)::value;
       const unsigned blocks = persistent_blocks(n);
-      const size_t need = (size_t)blocks * partial_floats(NL, 18, H, O) * 4;
+      const size_t need = ((size_t)blocks + kReduceSplit) * partial_floats(NL, 18, H, O) * 4;
       if (!ws || ws_bytes < need) return (int)FF_ERR_WORKSPACE;
       int r = launch_field_bwd<D, CK, BF, H, O, NL>(d, m, *grads, x, n, dsrc, mode, loss_scale,
                                                     (float*)ws, blocks, loss_sum, s);
@@ -295,7 +307,7 @@ size_t ff_linear_mat_bwd_workspace_bytes(const ff_mlp_desc* mlp, int64_t n) {
   if (check_mlp(mlp) || n <= 0) return 0;
   size_t bytes = 0;
   with_mlp(*mlp, mlp->in_dim, [&](auto NLi, auto Hi, auto Oi) {
-    bytes = (size_t)persistent_blocks(n) *
+    bytes = ((size_t)persistent_blocks(n) + kReduceSplit) *
             partial_floats(decltype(NLi)::value, 18, decltype(Hi)::value, decltype(Oi)::value) * 4;
     return 0;
   });
@@ -315,7 +327,7 @@ int ff_linear_mat_bwd(const ff_mlp_desc* mlp, const float* feats, int64_t n, con
   st = with_mlp(m, m.in_dim, [&](auto NLi, auto Hi, auto Oi) {
     constexpr int NL = decltype(NLi)::value, H = decltype(Hi)::value, O = decltype(Oi)::value;
     const unsigned blocks = persistent_blocks(n);
-    const size_t need = (size_t)blocks * partial_floats(NL, 18, H, O) * 4;
+    const size_t need = ((size_t)blocks + kReduceSplit) * partial_floats(NL, 18, H, O) * 4;
     if (!ws || ws_bytes < need) return (int)FF_ERR_WORKSPACE;
     int r = launch_mlp_bwd<H, O, NL>(m, feats, n, d_out, d_feats, (float*)ws, blocks, s);
     if (r) return r;

@@ -135,12 +135,13 @@ __device__ void gen_coeff_scatter(const ff_field_desc& fd, const ff_grads& g, co
 // Generic get_coding forward / backward (MODE 0 fwd, 1 bwd).
 template <int D, int XD, bool BF, int MODE>
 __global__ __launch_bounds__(256) void coding_generic_kernel(
-    ff_field_desc fd, ff_grads g, int ck, const float* __restrict__ x, int64_t n,
+    ff_field_desc fd, ff_grads gin, int ck, const float* __restrict__ x, int64_t n,
     float* __restrict__ feats, float* __restrict__ coeff, float* __restrict__ basis,
     const float* __restrict__ dfeat, const float* __restrict__ dcoeff,
     const float* __restrict__ dbasis) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const ff_grads g = local_grads(gin);
   const int C = fd.n_channels;
   float xv[XD];
   for (int a = 0; a < XD; ++a) xv[a] = x[p * XD + a];

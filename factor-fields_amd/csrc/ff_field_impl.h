@@ -248,6 +248,128 @@ __device__ __forceinline__ void coding_bwd(const ff_field_desc& fd, const ff_gra
   }
 }
 
+// ------------------------------------------------------------ coalesced scatter
+// Atomic throughput is bounded by cache-line requests, not by the lanes each
+// request carries, so the backward does not let every lane scatter its own
+// point: each lane stages its corners (texel, weight) and gradient vector in
+// a per-wave LDS area, then groups of d consecutive lanes add one texel's d
+// contiguous channels, the corners of a point one after the other (x
+// neighbours share a line).  Staging layout (floats): G [64*d] | W [64*K] |
+// T [64*K] (int bits).  Every lane of the wave must call wave_scatter.
+constexpr int kStageWords = 64 * 18 + 64 * 8 * 2;
+
+template <int K, int d>
+__device__ __forceinline__ void wave_scatter(float* stage, float* dst, const int* tex, const float* w,
+                                             const float* g, bool active) {
+  static_assert(d <= 64 && 64 * d + 2 * 64 * K <= kStageWords, "staging area too small");
+  const int lane = threadIdx.x & 63;
+  float* G = stage;
+  float* W = stage + 64 * d;
+  int* T = reinterpret_cast<int*>(stage + 64 * d + 64 * K);
+#pragma unroll
+  for (int c = 0; c < d; ++c) G[lane * d + c] = g[c];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    W[lane * K + k] = active ? w[k] : 0.f;
+    T[lane * K + k] = tex[k];
+  }
+  __syncthreads();
+  constexpr int PPI = 64 / d;  // (point, corner) pairs per instruction
+  const int sub = lane / d, c = lane - sub * d;
+#pragma unroll 2
+  for (int base = 0; base < 64 * K; base += PPI) {
+    const int q = base + sub;  // q = point * K + corner
+    if (sub < PPI && q < 64 * K) {
+      const float ww = W[q];
+      if (ww != 0.f) unsafeAtomicAdd(dst + (int64_t)T[q] * d + c, ww * G[(q / K) * d + c]);
+    }
+  }
+  __syncthreads();
+}
+
+// Level l of the basis: corners and scatter of grad[dl] (wave-wide).
+template <int D, int dl>
+__device__ __forceinline__ void basis_level_bwd_wave(const ff_field_desc& fd, float* dst, int l, const float* x,
+                                                     const float* grad, bool active, float* stage) {
+  const int R = fd.basis_reso[l];
+  float pos[D];
+  int Rs[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    pos[a] = unnorm_ac(map_coord(fd.mapping, x[a], fd.map_origin[a], fd.map_scale[l]), R);
+    Rs[a] = R;
+  }
+  if (fd.basis_interp == FF_INTERP_BILINEAR) {
+    Corners<D> cr;
+    linear_corners<D>(pos, Rs, cr);
+    wave_scatter<(1 << D), dl>(stage, dst, cr.tex, cr.w, grad, active);
+  } else {
+    const int t = nearest_texel<D>(pos, Rs);
+    const int tex[1] = {t >= 0 ? t : 0};
+    const float w[1] = {t >= 0 ? 1.f : 0.f};
+    wave_scatter<1, dl>(stage, dst, tex, w, grad, active);
+  }
+}
+
+// Border-padded C-channel table at normalised pts (wave-wide scatter).
+template <int ND, int C>
+__device__ __forceinline__ void grid_scatter_wave(float* dst, const int* R, const float* pts, int interp,
+                                                  const float* grad, bool active, float* stage) {
+  float pos[ND];
+#pragma unroll
+  for (int a = 0; a < ND; ++a) pos[a] = unnorm_border(pts[a], R[a]);
+  if (interp == FF_INTERP_BILINEAR) {
+    Corners<ND> cr;
+    linear_corners<ND>(pos, R, cr);
+    wave_scatter<(1 << ND), C>(stage, dst, cr.tex, cr.w, grad, active);
+  } else {
+    const int t = nearest_texel<ND>(pos, R);
+    const int tex[1] = {t >= 0 ? t : 0};
+    const float w[1] = {t >= 0 ? 1.f : 0.f};
+    wave_scatter<1, C>(stage, dst, tex, w, grad, active);
+  }
+}
+
+// coding_bwd for a whole wave (every lane calls it; inactive lanes add nothing).
+template <int D, int CK, bool BF, class S>
+__device__ __forceinline__ void coding_bwd_wave(const ff_field_desc& fd, const ff_grads& g, const float* x,
+                                                const float* dbas, const float* dcof, bool active,
+                                                float* stage) {
+  constexpr int C = S::C;
+  if (fd.basis_kind == FF_BASIS_GRID) {
+    static_for<0, S::L>([&](auto LI) {
+      constexpr int l = decltype(LI)::value;
+      basis_level_bwd_wave<D, S::dim(l)>(fd, g.basis[l], l, x, dbas + S::off(l), active, stage);
+    });
+  }
+  constexpr int XD = XDim<D, CK>::value;
+  if constexpr (CK == CK_GRID2 || CK == CK_GRID3) {
+    constexpr int ND = CK == CK_GRID2 ? 2 : 3;
+    float pts[XD];
+    coeff_pts<XD>(fd, x, pts);
+    grid_scatter_wave<ND, C>(g.coeff[0], fd.coeff_reso, pts, fd.coeff_interp, dcof, active, stage);
+  } else if constexpr (CK == CK_VM) {
+    float pts[3];
+    coeff_pts<3>(fd, x, pts);
+#pragma unroll 1
+    for (int m = 0; m < 3; ++m) {
+      float pv[C], lv[C], t[C];
+      vm_terms<C, BF>(fd, m, pts, pv, lv);
+      const int a0 = kMatMode[m][0], a1 = kMatMode[m][1], av = kVecMode[m];
+      const int Rp[2] = {fd.coeff_reso[a0], fd.coeff_reso[a1]};
+      const float pp[2] = {pts[a0], pts[a1]};
+#pragma unroll
+      for (int c = 0; c < C; ++c) t[c] = dcof[c] * lv[c];
+      grid_scatter_wave<2, C>(g.coeff[m], Rp, pp, fd.coeff_interp, t, active, stage);
+      const int Rl[2] = {1, fd.coeff_reso[av]};
+      const float pl[2] = {0.f, pts[av]};
+#pragma unroll
+      for (int c = 0; c < C; ++c) t[c] = dcof[c] * pv[c];
+      grid_scatter_wave<2, C>(g.coeff[3 + m], Rl, pl, fd.coeff_interp, t, active, stage);
+    }
+  }
+}
+
 // feats = basis * coeff, with the 'none' variants of get_coding.
 template <int C, int CK>
 __device__ __forceinline__ void combine(const ff_field_desc& fd, const float* bas, const float* cof,
@@ -511,33 +633,36 @@ __global__ __launch_bounds__(256) void coding_fwd_kernel(ff_field_desc fd, const
 }
 
 template <int D, int CK, bool BF, class S>
-__global__ __launch_bounds__(256) void coding_bwd_kernel(ff_field_desc fd, ff_grads g,
-                                                         const float* __restrict__ x, int64_t n,
-                                                         const float* __restrict__ dfeat,
-                                                         const float* __restrict__ dcoeff,
-                                                         const float* __restrict__ dbasis) {
+__global__ __launch_bounds__(64) void coding_bwd_kernel(ff_field_desc fd, ff_grads g,
+                                                        const float* __restrict__ x, int64_t n,
+                                                        const float* __restrict__ dfeat,
+                                                        const float* __restrict__ dcoeff,
+                                                        const float* __restrict__ dbasis) {
   constexpr int C = S::C;
   constexpr int XD = XDim<D, CK>::value;
+  __shared__ float stage[kStageWords];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  const bool active = p < n;
+  const int64_t pc = active ? p : n - 1;
   float xv[XD];
 #pragma unroll
-  for (int a = 0; a < XD; ++a) xv[a] = x[p * XD + a];
+  for (int a = 0; a < XD; ++a) xv[a] = x[pc * XD + a];
   float bas[C], cof[C], df[C], db[C], dc[C];
+  const ff_grads gl = local_grads(g);
   coding_fwd<D, CK, BF, S>(fd, xv, bas, cof);
 #pragma unroll
-  for (int c = 0; c < C; ++c) df[c] = dfeat ? dfeat[p * C + c] : 0.f;
+  for (int c = 0; c < C; ++c) df[c] = dfeat ? dfeat[pc * C + c] : 0.f;
   combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
   // the returned coeff aliases basis when the coefficient is 'none' (and vice versa)
 #pragma unroll
   for (int c = 0; c < C; ++c) {
-    const float gc = dcoeff ? dcoeff[p * C + c] : 0.f;
-    const float gb = dbasis ? dbasis[p * C + c] : 0.f;
+    const float gc = dcoeff ? dcoeff[pc * C + c] : 0.f;
+    const float gb = dbasis ? dbasis[pc * C + c] : 0.f;
     if (CK == CK_NONE) db[c] += gc + gb;
     else if (fd.basis_kind == FF_BASIS_NONE) dc[c] += gc + gb;
     else { dc[c] += gc; db[c] += gb; }
   }
-  coding_bwd<D, CK, BF, S>(fd, g, xv, db, dc);
+  coding_bwd_wave<D, CK, BF, S>(fd, gl, xv, db, dc, active, stage);
 }
 
 // Fused get_coding + linear_mat forward.
@@ -572,13 +697,14 @@ __global__ __launch_bounds__(64) void field_bwd_kernel(ff_field_desc fd, ff_mlp_
   constexpr int C = S::C;
   constexpr int XD = XDim<D, CK>::value;
   using Acc = MlpAcc<C, H, O, NL>;
-  __shared__ float lds[NL == 2 ? kWave * 65 : 1];
+  __shared__ float lds[kWave * 65 > kStageWords ? kWave * 65 : kStageWords];
   Acc acc;
   acc.zero();
   float lsum = 0.f;
   const int lane = threadIdx.x;
   const int64_t ntiles = (n + kWave - 1) / kWave;
   const float grad_scale = (float)(1.0 / ((double)n * O)) * loss_scale;
+  const ff_grads gl = local_grads(g);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t p = tile * kWave + lane;
     const bool active = p < n;
@@ -590,11 +716,9 @@ __global__ __launch_bounds__(64) void field_bwd_kernel(ff_field_desc fd, ff_mlp_
     coding_fwd<D, CK, BF, S>(fd, xv, bas, cof);
     combine<C, CK>(fd, bas, cof, f);
     mlp_train_tile<C, H, O, NL>(m, acc, lds, f, active, pc, dsrc, mode, grad_scale, lsum, df);
-    if (active) {
-      float db[C], dc[C];
-      combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
-      coding_bwd<D, CK, BF, S>(fd, g, xv, db, dc);
-    }
+    float db[C], dc[C];
+    combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
+    coding_bwd_wave<D, CK, BF, S>(fd, gl, xv, db, dc, active, lds);
   }
   mlp_acc_store<C, H, O, NL>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
   if (mode != 0) {
@@ -665,7 +789,7 @@ int launch_coding_fwd(const ff_field_desc& d, const float* x, int64_t n, float* 
 template <int D, int CK, bool BF>
 int launch_coding_bwd(const ff_field_desc& d, const ff_grads& g, const float* x, int64_t n,
                       const float* df, const float* dc, const float* db, hipStream_t s) {
-  hipLaunchKernelGGL((coding_bwd_kernel<D, CK, BF, Spec18>), dim3(blocks_for(n, 256)), dim3(256), 0,
+  hipLaunchKernelGGL((coding_bwd_kernel<D, CK, BF, Spec18>), dim3(blocks_for(n, 64)), dim3(64), 0,
                      s, d, g, x, n, df, dc, db);
   return launch_status();
 }

@@ -1,0 +1,399 @@
+// Learned proposal sampler (gfx950, wave64): one wavefront per ray, one lane
+// per proposal sample (64 per ray).
+//
+// Forward: per lane, trilinear sample of the 8-channel proposal grid, MLP
+// 8 -> 32 -> 1 (weights through the scalar cache), softplus density; the
+// proposal weights come from a multiplicative prefix scan across the lanes
+// (as in compositing, models/FactorFields.py:82-88); sample_pdf
+// (dataLoader/ray_utils.py:132-174) is an additive prefix scan for the CDF
+// and, for each fine sample, a binary search over the lane-held CDF with
+// lane shuffles.  Fine samples are written ray-major like the occupancy
+// sampler's output, so the field and compositing kernels consume either.
+//
+// Backward: recomputes the proposal forward, evaluates the interlevel bound
+// loss against the field's fine weights, and back-propagates through the
+// proposal compositing (reverse affine scan), the MLP and the grid.
+#include "ff_common.h"
+
+namespace ff {
+
+int launch_scan_offsets(int32_t* off, int64_t B, hipStream_t s);  // ff_render.hip
+
+typedef const __attribute__((address_space(4))) float* pcfp;
+constexpr int kPH = 32;  // proposal MLP hidden width
+constexpr int kPC = 8;   // proposal grid channels
+
+struct PropRay {
+  float o[3], d[3];
+  float tn, tf;
+  bool hit;
+};
+
+__device__ __forceinline__ void prop_ray(const ff_proposal_desc& p, const float* __restrict__ rays, int64_t b,
+                                         PropRay& r) {
+  float tn = -INFINITY, tf = INFINITY;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    r.o[a] = rays[b * 6 + a];
+    r.d[a] = rays[b * 6 + 3 + a];
+    const float vec = r.d[a] == 0.f ? 1e-6f : r.d[a];
+    const float ta = (p.aabb_max[a] - r.o[a]) / vec;
+    const float tb = (p.aabb_min[a] - r.o[a]) / vec;
+    tn = fmaxf(tn, fminf(ta, tb));
+    tf = fminf(tf, fmaxf(ta, tb));
+  }
+  r.tn = fmaxf(tn, p.near);
+  r.tf = tf;
+  r.hit = tf > r.tn;
+}
+
+__device__ __forceinline__ float prop_softplus(float x) { return x > 20.f ? x : log1pf(expf(x)); }
+__device__ __forceinline__ float prop_softplus_grad(float x) {
+  if (x > 20.f) return 1.f;
+  const float z = expf(x);
+  return z / (z + 1.f);
+}
+
+// Proposal density at x; keeps the features, hidden activations and corners for
+// the backward pass.
+template <bool BF>
+__device__ __forceinline__ float prop_density(const ff_proposal_desc& p, const float* x, float* f, float* h,
+                                              float& pre, Corners<3>& cr) {
+  float pos[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float inv = 2.f / (p.aabb_max[a] - p.aabb_min[a]);
+    pos[a] = unnorm_border((x[a] - p.aabb_min[a]) * inv - 1.f, p.reso[a]);
+  }
+  linear_corners<3>(pos, p.reso, cr);
+#pragma unroll
+  for (int c = 0; c < kPC; ++c) f[c] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    float v[kPC];
+    load_ch<kPC, BF>(p.grid, (int64_t)cr.tex[k] * kPC, v);
+#pragma unroll
+    for (int c = 0; c < kPC; ++c) f[c] = fmaf(v[c], cr.w[k], f[c]);
+  }
+  const pcfp w0 = (pcfp)p.w0;
+  const pcfp b0 = (pcfp)p.b0;
+  const pcfp w1 = (pcfp)p.w1;
+  float o = 0.f;
+#pragma unroll
+  for (int i = 0; i < kPH; ++i) {
+    float a = b0[i];
+#pragma unroll
+    for (int c = 0; c < kPC; ++c) a = fmaf(w0[i * kPC + c], f[c], a);
+    h[i] = fmaxf(a, 0.f);
+    o = fmaf(w1[i], h[i], o);
+  }
+  pre = o + p.density_shift;
+  return prop_softplus(pre);
+}
+
+__device__ __forceinline__ float wave_incl_scan_mul(float v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const float t = __shfl_up(v, d, 64);
+    if (lane >= d) v = v * t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ float wave_incl_scan_add(float v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const float t = __shfl_up(v, d, 64);
+    if (lane >= d) v = v + t;
+  }
+  return v;
+}
+
+// Everything the forward computes for one ray (lane j = proposal sample j).
+struct PropState {
+  float s, s_next;     // bin edges of lane j
+  float t, x[3];       // proposal sample
+  float sigma, pre, alpha, f, T, w, delta, ex;
+  float feat[kPC], h[kPH];
+  Corners<3> cr;
+};
+
+template <bool BF>
+__device__ __forceinline__ void prop_forward(const ff_proposal_desc& p, const PropRay& r,
+                                             const float* __restrict__ prop_u, int64_t b, int lane,
+                                             PropState& st) {
+  const float span = r.tf - r.tn;
+  st.s = r.tn + span * ((float)lane / 64.f);
+  st.s_next = lane == 63 ? r.tf : r.tn + span * ((float)(lane + 1) / 64.f);
+  const float u = prop_u ? prop_u[b * 64 + lane] : 0.5f;
+  st.t = st.s + u * (st.s_next - st.s);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) st.x[a] = r.o[a] + r.d[a] * st.t;
+  st.sigma = prop_density<BF>(p, st.x, st.feat, st.h, st.pre, st.cr);
+  st.delta = (st.s_next - st.s) * p.distance_scale;
+  st.ex = expf(-st.sigma * st.delta);
+  st.alpha = 1.f - st.ex;
+  st.f = (1.f - st.alpha) + 1e-10f;
+  const float incl = wave_incl_scan_mul(st.f, lane);
+  float excl = __shfl_up(incl, 1, 64);
+  if (lane == 0) excl = 1.f;
+  st.T = excl;
+  st.w = st.alpha * st.T;
+}
+
+// Bin edge i (0..64) of the ray: lanes hold edges 0..63, edge 64 is t_far.
+__device__ __forceinline__ float edge_at(float s_lane, float tf, int i) {
+  const float v = __shfl(s_lane, i < 64 ? i : 63, 64);
+  return i < 64 ? v : tf;
+}
+
+template <bool BF>
+__global__ __launch_bounds__(256) void proposal_sample_kernel(ff_proposal_desc p, const float* __restrict__ rays,
+                                                              const float* __restrict__ prop_u,
+                                                              const float* __restrict__ fine_u, int64_t B,
+                                                              int pass, ff_samples out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (b >= B) return;
+  PropRay r;
+  prop_ray(p, rays, b, r);
+  const int nf = p.n_fine;
+  if (pass == 0) {
+    if (lane == 0) out.ray_off[b + 1] = r.hit ? nf : 0;
+    return;
+  }
+  if (!r.hit) return;
+  PropState st;
+  prop_forward<BF>(p, r, prop_u, b, lane, st);
+  // sample_pdf: weights + 1e-5, normalised, inclusive CDF (lane j holds cdf[j+1])
+  const float wp = st.w + 1e-5f;
+  const float total = wave_sum(wp);
+  const float cdf = wave_incl_scan_add(wp / total, lane);
+  float u;
+  if (fine_u) u = ((float)lane + fine_u[b * nf + (lane < nf ? lane : 0)]) / (float)nf;
+  else u = nf > 1 ? (float)lane / (float)(nf - 1) : 0.f;
+  // inds = searchsorted(cdf, u, right=True) = #{i in 0..64 : cdf[i] <= u}, cdf[0] = 0
+  int lo = 0, hi = 65;
+#pragma unroll
+  for (int it = 0; it < 7; ++it) {
+    const int mid = (lo + hi) >> 1;
+    const float cm = __shfl(cdf, mid > 0 ? mid - 1 : 0, 64);
+    const float v = mid == 0 ? 0.f : cm;
+    const bool le = (lo < hi) && v <= u;
+    lo = le ? mid + 1 : lo;
+    hi = (lo < hi && !le) ? mid : hi;
+  }
+  const int inds = lo;
+  const int below = inds - 1 > 0 ? inds - 1 : 0;
+  const int above = inds < 64 ? inds : 64;
+  const float c0v = __shfl(cdf, below > 0 ? below - 1 : 0, 64);
+  const float c1v = __shfl(cdf, above > 0 ? above - 1 : 0, 64);
+  const float cdf0 = below == 0 ? 0.f : c0v;
+  const float cdf1 = above == 0 ? 0.f : c1v;
+  const float bin0 = edge_at(st.s, r.tf, below);
+  const float bin1 = edge_at(st.s, r.tf, above);
+  float denom = cdf1 - cdf0;
+  denom = denom < 1e-5f ? 1.f : denom;
+  const float tt = (u - cdf0) / denom;
+  const float tk = bin0 + tt * (bin1 - bin0);
+  const float tnext = __shfl_down(tk, 1, 64);
+  if (lane < nf) {
+    const int64_t pos = (int64_t)out.ray_off[b] + lane;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out.xyz[pos * 3 + a] = r.o[a] + r.d[a] * tk;
+    out.z[pos] = tk;
+    out.dist[pos] = (lane + 1 < nf ? tnext - tk : 0.f) * p.distance_scale;
+    out.ray[pos] = (int32_t)b;
+    out.idx[pos] = lane;
+  }
+}
+
+constexpr int kPropParams = kPH * kPC + kPH + kPH;  // dW0, db0, dW1
+
+template <bool BF>
+__global__ __launch_bounds__(256) void proposal_bwd_kernel(ff_proposal_desc p, const float* __restrict__ rays,
+                                                           const float* __restrict__ prop_u, int64_t B,
+                                                           ff_samples fine, const float* __restrict__ weights,
+                                                           float loss_scale, ff_proposal_grads g,
+                                                           float* __restrict__ loss_sum) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  float* grid_g = g.grid;
+  if (g.replicas > 1) grid_g += (int64_t)(xcc_id() % g.replicas) * g.replica_stride;
+  // lane-distributed decoder-gradient accumulators: value i lives in lane i % 64, slot i / 64
+  constexpr int kSlots = (kPropParams + 63) / 64;
+  float acc[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) acc[k] = 0.f;
+  float lsum = 0.f;
+  const float scale = loss_scale / (float)B;
+  for (int64_t b = wave; b < B; b += nwaves) {
+    const int cnt = fine.ray_off[b + 1] - fine.ray_off[b];
+    if (cnt < 2) continue;
+    PropRay r;
+    prop_ray(p, rays, b, r);
+    PropState st;
+    prop_forward<BF>(p, r, prop_u, b, lane, st);
+    const int64_t f0 = fine.ray_off[b];
+    const float tf_k = lane < cnt ? fine.z[f0 + lane] : 0.f;
+    const float wf_k = lane < cnt ? weights[f0 + lane] : 0.f;
+    const float wincl = wave_incl_scan_add(st.w, lane);
+    const float wexcl = wincl - st.w;
+    float dw = 0.f;
+    for (int k = 0; k + 1 < cnt; ++k) {
+      const float t0 = __shfl(tf_k, k, 64);
+      const float t1 = __shfl(tf_k, k + 1, 64);
+      const float wf = __shfl(wf_k, k, 64);
+      const uint64_t m_lo = __ballot(lane >= 1 && st.s <= t0);
+      int j_lo = __popcll(m_lo) + (r.tf <= t0 ? 1 : 0);
+      int j_hi = __popcll(__ballot(st.s < t1)) - 1;
+      j_lo = j_lo > 63 ? 63 : j_lo;
+      j_hi = j_hi < 0 ? 0 : (j_hi > 63 ? 63 : j_hi);
+      if (j_hi < j_lo) j_hi = j_lo;
+      const float bound = __shfl(wincl, j_hi, 64) - __shfl(wexcl, j_lo, 64);
+      const float gap = fmaxf(wf - bound, 0.f);
+      const float inv = 1.f / (wf + 1e-7f);
+      lsum += (lane == 0) ? gap * gap * inv : 0.f;
+      const float gk = -2.f * gap * inv * scale;
+      if (lane >= j_lo && lane <= j_hi) dw += gk;
+    }
+    // proposal compositing backward: reverse affine scan
+    float A = st.f, Bv = dw * st.alpha;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const float A2 = __shfl_down(A, d, 64);
+      const float B2 = __shfl_down(Bv, d, 64);
+      if (lane + d < 64) {
+        Bv = Bv + A * B2;
+        A = A * A2;
+      }
+    }
+    float R = __shfl_down(Bv, 1, 64);
+    if (lane == 63) R = 0.f;
+    const float dalpha = st.T * (dw - R);
+    const float dsig = dalpha * (st.delta * st.ex);
+    const float dout = dsig * prop_softplus_grad(st.pre);
+    // MLP backward
+    const pcfp w0 = (pcfp)p.w0;
+    const pcfp w1 = (pcfp)p.w1;
+    float df[kPC];
+#pragma unroll
+    for (int c = 0; c < kPC; ++c) df[c] = 0.f;
+#pragma unroll
+    for (int i = 0; i < kPH; ++i) {
+      const float dh = st.h[i] > 0.f ? w1[i] * dout : 0.f;
+#pragma unroll
+      for (int c = 0; c < kPC; ++c) df[c] = fmaf(w0[i * kPC + c], dh, df[c]);
+      // dW0[i][c] += dh f[c]; db0[i] += dh; dW1[i] += dout h[i]
+#pragma unroll
+      for (int c = 0; c < kPC; ++c) {
+        const float s = wave_sum(dh * st.feat[c]);
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int idx = i * kPC + c;
+        if (lane == (idx & 63)) acc[idx >> 6] += s;
+      }
+      {
+        const float s = wave_sum(dh);
+        const int idx = kPH * kPC + i;
+        if (lane == (idx & 63)) acc[idx >> 6] += s;
+      }
+      {
+        const float s = wave_sum(dout * st.h[i]);
+        const int idx = kPH * kPC + kPH + i;
+        if (lane == (idx & 63)) acc[idx >> 6] += s;
+      }
+    }
+    // grid scatter
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (st.cr.w[k] != 0.f) {
+        float* q = grid_g + (int64_t)st.cr.tex[k] * kPC;
+#pragma unroll
+        for (int c = 0; c < kPC; ++c) unsafeAtomicAdd(q + c, st.cr.w[k] * df[c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const int idx = k * 64 + lane;
+    if (idx < kPropParams && acc[k] != 0.f) {
+      float* dst = idx < kPH * kPC ? g.w0 + idx : (idx < kPH * kPC + kPH ? g.b0 + (idx - kPH * kPC)
+                                                                         : g.w1 + (idx - kPH * kPC - kPH));
+      unsafeAtomicAdd(dst, acc[k]);
+    }
+  }
+  lsum = wave_sum(lsum);
+  if (lane == 0 && loss_sum) unsafeAtomicAdd(loss_sum, lsum * loss_scale / (float)B);
+}
+
+int prop_check(const ff_proposal_desc* p) {
+  if (!p || !p->grid || !p->w0 || !p->b0 || !p->w1) return FF_ERR_ARG;
+  for (int a = 0; a < 3; ++a)
+    if (p->reso[a] < 1 || !(p->aabb_max[a] > p->aabb_min[a])) return FF_ERR_ARG;
+  if (p->n_fine < 2 || p->n_fine > 64) return FF_ERR_ARG;
+  if (p->table_dtype != FF_F32 && p->table_dtype != FF_BF16) return FF_ERR_ARG;
+  return FF_OK;
+}
+
+}  // namespace ff
+
+using namespace ff;
+
+extern "C" {
+
+int ff_proposal_sample(const ff_proposal_desc* p, const float* rays, const float* prop_u,
+                       const float* fine_u, int64_t n_rays, const ff_samples* out, void* stream) {
+  int st = prop_check(p);
+  if (st) return st;
+  if (!out || !out->ray_off || n_rays < 0 || (n_rays > 0 && !rays)) return FF_ERR_ARG;
+  if (!out->xyz || !out->z || !out->dist || !out->ray || !out->idx) return FF_ERR_ARG;
+  if (out->cap < n_rays * (int64_t)p->n_fine) return FF_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (n_rays == 0) {
+    hipMemsetAsync(out->ray_off, 0, sizeof(int32_t), s);
+    return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
+  }
+  const unsigned blocks = (unsigned)((n_rays * 64 + 255) / 256);
+  const ff_proposal_desc pd = *p;
+  auto go = [&](auto BFi) {
+    constexpr bool BF = decltype(BFi)::value;
+    hipLaunchKernelGGL((proposal_sample_kernel<BF>), dim3(blocks), dim3(256), 0, s, pd, rays, prop_u, fine_u,
+                       n_rays, 0, *out);
+    int r = launch_scan_offsets(out->ray_off, n_rays, s);
+    if (r) return r;
+    hipLaunchKernelGGL((proposal_sample_kernel<BF>), dim3(blocks), dim3(256), 0, s, pd, rays, prop_u, fine_u,
+                       n_rays, 1, *out);
+    return hipGetLastError() == hipSuccess ? (int)FF_OK : (int)FF_ERR_LAUNCH;
+  };
+  return p->table_dtype == FF_BF16 ? go(std::true_type{}) : go(std::false_type{});
+}
+
+int ff_proposal_bwd(const ff_proposal_desc* p, const float* rays, const float* prop_u, int64_t n_rays,
+                    const ff_samples* fine, const float* weights, float loss_scale,
+                    const ff_proposal_grads* grads, float* loss_sum, void* stream) {
+  int st = prop_check(p);
+  if (st) return st;
+  if (!fine || !fine->ray_off || !fine->z || !weights || !grads || !grads->grid || !grads->w0 || !grads->b0 ||
+      !grads->w1 || n_rays < 0 || (n_rays > 0 && !rays))
+    return FF_ERR_ARG;
+  if (n_rays == 0) return FF_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int cu = 256, dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int64_t waves = n_rays < (int64_t)cu * 8 ? n_rays : (int64_t)cu * 8;
+  const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
+  const ff_proposal_desc pd = *p;
+  const ff_proposal_grads gd = *grads;
+  const ff_samples fs = *fine;
+  if (p->table_dtype == FF_BF16)
+    hipLaunchKernelGGL((proposal_bwd_kernel<true>), dim3(blocks), dim3(256), 0, s, pd, rays, prop_u, n_rays, fs,
+                       weights, loss_scale, gd, loss_sum);
+  else
+    hipLaunchKernelGGL((proposal_bwd_kernel<false>), dim3(blocks), dim3(256), 0, s, pd, rays, prop_u, n_rays, fs,
+                       weights, loss_scale, gd, loss_sum);
+  return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
+}
+
+}  // extern "C"

@@ -322,6 +322,32 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
   }
 }
 
+// alpha/times clamped to [0,1], 3x3x3 max (out-of-range neighbours ignored), >= thres.
+__global__ __launch_bounds__(256) void occupancy_pool_kernel(const float* __restrict__ alpha, int D, int H,
+                                                             int W, float times, float thres,
+                                                             uint8_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)D * H * W;
+  if (i >= n) return;
+  const int x = (int)(i % W), y = (int)((i / W) % H), z = (int)(i / ((int64_t)W * H));
+  float m = -INFINITY;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int zz = z + dz, yy = y + dy, xx = x + dx;
+        if (zz < 0 || zz >= D || yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
+        float v = alpha[((int64_t)zz * H + yy) * W + xx] / times;
+        v = fminf(fmaxf(v, 0.f), 1.f);
+        m = fmaxf(m, v);
+      }
+  out[i] = m >= thres ? 1 : 0;
+}
+
+int launch_scan_offsets(int32_t* off, int64_t B, hipStream_t s) {
+  hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(1024), 0, s, off, B);
+  return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
+}
+
 }  // namespace ff
 
 using namespace ff;
@@ -378,6 +404,15 @@ int ff_composite_bwd(const ff_samples* s, int64_t n_rays, const float* sigma, co
   hipLaunchKernelGGL(composite_bwd_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, *s, n_rays,
                      sigma, rgb, white_bg, weight_thres, rgb_raw, d_rgb_map, target, loss_scale,
                      loss_sum, d_sample);
+  return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
+}
+
+int ff_occupancy_pool(const float* alpha, int32_t D, int32_t H, int32_t W, float times, float thres,
+                      uint8_t* out, void* stream) {
+  if (!alpha || !out || D < 1 || H < 1 || W < 1 || !(times > 0.f)) return FF_ERR_ARG;
+  const int64_t n = (int64_t)D * H * W;
+  hipLaunchKernelGGL(occupancy_pool_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, alpha, D, H, W, times, thres, out);
   return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
 }
 

@@ -98,6 +98,7 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff
   const int lane = threadIdx.x;
   const int64_t n = s.ray_off[n_rays];
   const int64_t ntiles = (n + kWave - 1) / kWave;
+  const ff_grads gl = local_grads(g);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t i = tile * kWave + lane;
     const bool active = i < n;
@@ -126,13 +127,42 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff
           }
         },
         df);
-    if (active) {
-      float db[C], dc[C];
-      combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
-      coding_bwd<3, CK, BF, Spec18>(fd, g, xv, db, dc);
-    }
+    float db[C], dc[C];
+    combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
+    coding_bwd_wave<3, CK, BF, Spec18>(fd, gl, xv, db, dc, active, lds);
   }
   mlp_acc_store<C, 64, 28, 2>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
+}
+
+// compute_alpha (models/FactorFields.py:710-727) accumulated into alpha.
+template <int CK, bool BF>
+__global__ __launch_bounds__(256) void nerf_alpha_kernel(ff_field_desc fd, ff_mlp_desc m, ff_render_head hd,
+                                                         ff_ray_sampler occ, const float* __restrict__ xyz,
+                                                         int64_t n, float length,
+                                                         float* __restrict__ alpha) {
+  constexpr int C = Spec18::C;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float xv[3] = {xyz[i * 3 + 0], xyz[i * 3 + 1], xyz[i * 3 + 2]};
+  if (occ.occ) {
+    float pos[3];
+    const int R[3] = {occ.occ_reso[0], occ.occ_reso[1], occ.occ_reso[2]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pos[a] = unnorm_ac((xv[a] - occ.occ_origin[a]) * occ.occ_inv[a] - 1.f, R[a]);
+    Corners<3> cr;
+    linear_corners<3>(pos, R, cr);
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc = acc + (occ.occ[cr.tex[k]] ? cr.w[k] : 0.f);
+    if (!(acc > 0.f)) return;   // sigma = 0 -> alpha += 0
+  }
+  float bas[C], cof[C], f[C], h[64], o[28];
+  coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
+  combine<C, CK>(fd, bas, cof, f);
+  mlp_fwd<C, 64, 28, 2>(m, f, h, o);
+  const float x0 = o[0] + hd.density_shift;
+  const float sg = hd.density_act == 0 ? softplusf(x0) : fmaxf(x0, 0.f);
+  alpha[i] += 1.f - expf(-sg * length);
 }
 
 namespace {
@@ -210,10 +240,35 @@ int ff_render_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
   });
 }
 
+int ff_render_alpha(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
+                    const ff_ray_sampler* sampler, const float* xyz, int64_t n, float length,
+                    float* alpha_accum, void* stream) {
+  if (!desc || !mlp || !head || n < 0 || (n > 0 && (!xyz || !alpha_accum))) return FF_ERR_ARG;
+  if (!nerf_supported(*desc, *mlp)) return FF_ERR_UNSUPPORTED;
+  if (n == 0) return FF_OK;
+  ff_ray_sampler occ{};
+  if (sampler && sampler->occ) {
+    occ = *sampler;
+    for (int a = 0; a < 3; ++a)
+      if (occ.occ_reso[a] < 2) return FF_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const ff_field_desc d = *desc;
+  const ff_mlp_desc m = *mlp;
+  const ff_render_head h = *head;
+  return with_nerf(d, [&](auto CKi, auto BFi) {
+    constexpr int CK = decltype(CKi)::value;
+    constexpr bool BF = decltype(BFi)::value;
+    hipLaunchKernelGGL((nerf_alpha_kernel<CK, BF>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, m,
+                       h, occ, xyz, n, length, alpha_accum);
+    return rf_status();
+  });
+}
+
 size_t ff_render_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp,
                                            int64_t cap) {
   if (!desc || !mlp || cap <= 0 || !nerf_supported(*desc, *mlp)) return 0;
-  return (size_t)rf_blocks(cap, 64, 16) * kNerfPartial * 4;
+  return ((size_t)rf_blocks(cap, 64, 16) + 32) * kNerfPartial * 4;
 }
 
 int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
@@ -231,7 +286,7 @@ int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
   if (!grads->w0 || !grads->b0 || !grads->w1) return FF_ERR_ARG;
   if (s->cap <= 0 || n_rays == 0) return FF_OK;
   const unsigned blocks = rf_blocks(s->cap, 64, 16);
-  if (!ws || ws_bytes < (size_t)blocks * kNerfPartial * 4) return FF_ERR_WORKSPACE;
+  if (!ws || ws_bytes < ((size_t)blocks + 32) * kNerfPartial * 4) return FF_ERR_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   const ff_field_desc d = *desc;
   const ff_mlp_desc m = *mlp;

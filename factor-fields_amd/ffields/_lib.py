@@ -49,12 +49,14 @@ class MlpDesc(ctypes.Structure):
 
 class Grads(ctypes.Structure):
     _fields_ = [("basis", c_vp * FF_MAX_LEVELS), ("coeff", c_vp * 6),
-                ("w0", c_vp), ("b0", c_vp), ("w1", c_vp)]
+                ("w0", c_vp), ("b0", c_vp), ("w1", c_vp), ("replicas", c_i32),
+                ("replica_stride", ctypes.c_int64)]
 
 
 class AdamTensor(ctypes.Structure):
     _fields_ = [("param", c_vp), ("grad", c_vp), ("exp_avg", c_vp), ("exp_avg_sq", c_vp),
-                ("shadow_bf16", c_vp), ("n", ctypes.c_int64), ("lr", c_f32)]
+                ("shadow_bf16", c_vp), ("n", ctypes.c_int64), ("lr", c_f32),
+                ("grad_replicas", c_i32), ("grad_stride", ctypes.c_int64)]
 
 
 class RaySampler(ctypes.Structure):
@@ -70,6 +72,17 @@ class Samples(ctypes.Structure):
 
 class RenderHead(ctypes.Structure):
     _fields_ = [("density_act", c_i32), ("density_shift", c_f32)]
+
+
+class ProposalDesc(ctypes.Structure):
+    _fields_ = [("reso", c_i32 * 3), ("table_dtype", c_i32), ("grid", c_vp), ("w0", c_vp), ("b0", c_vp),
+                ("w1", c_vp), ("aabb_min", c_f32 * 3), ("aabb_max", c_f32 * 3), ("density_shift", c_f32),
+                ("distance_scale", c_f32), ("near", c_f32), ("n_fine", c_i32)]
+
+
+class ProposalGrads(ctypes.Structure):
+    _fields_ = [("grid", c_vp), ("w0", c_vp), ("b0", c_vp), ("w1", c_vp), ("replicas", c_i32),
+                ("replica_stride", ctypes.c_int64)]
 
 
 _lib = None
@@ -100,12 +113,19 @@ def _declare(lib):
         "ff_adam_multi": (ctypes.c_int, [P(AdamTensor), ctypes.c_int, c_f32, c_f32, c_f32, I64,
                                          ctypes.c_int, c_vp]),
         "ff_cast_bf16": (ctypes.c_int, [c_vp, c_vp, I64, c_vp]),
+        "ff_reduce_replicas": (ctypes.c_int, [c_vp, c_i32, I64, I64, c_vp]),
         "ff_sample_rays": (ctypes.c_int, [P(RaySampler), c_vp, c_vp, I64, P(Samples), c_vp]),
         "ff_render_field_fwd": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), P(RenderHead), c_vp, I64,
                                                P(Samples), c_vp, c_vp, c_vp]),
         "ff_render_field_bwd_workspace_bytes": (sz, [P(FieldDesc), P(MlpDesc), I64]),
         "ff_render_field_bwd": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), P(RenderHead), c_vp, I64,
                                                P(Samples), c_vp, P(Grads), c_vp, sz, c_vp]),
+        "ff_render_alpha": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), P(RenderHead), P(RaySampler), c_vp,
+                                           I64, c_f32, c_vp, c_vp]),
+        "ff_occupancy_pool": (ctypes.c_int, [c_vp, c_i32, c_i32, c_i32, c_f32, c_f32, c_vp, c_vp]),
+        "ff_proposal_sample": (ctypes.c_int, [P(ProposalDesc), c_vp, c_vp, c_vp, I64, P(Samples), c_vp]),
+        "ff_proposal_bwd": (ctypes.c_int, [P(ProposalDesc), c_vp, c_vp, I64, P(Samples), c_vp, c_f32,
+                                           P(ProposalGrads), c_vp, c_vp]),
         "ff_composite_fwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp]),
         "ff_composite_bwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,

@@ -254,24 +254,57 @@ class FactorFields(RenderMixin, nn.Module):
             return nn.functional.softplus(density_features + r.density_shift)
         return nn.functional.relu(density_features + r.density_shift)
 
-    # ------------------------------------------------ fused regression step
+    # ------------------------------------------------ fused training steps
+    grad_replicas = 8   # one gradient copy per XCD for the fused training steps
+
     def _ensure_grads(self):
         for p in self.parameters():
             if p.grad is None:
                 p.grad = torch.zeros_like(p, memory_format=torch.preserve_format)
 
-    def regression_step(self, x, target, loss="mse", loss_scale=1.0, loss_sum=None):
-        """Forward, loss and backward of one regression batch in one kernel
-        (scripts/sdf_regression.ipynb:369-379).  Gradients are accumulated
-        into each parameter's ``.grad``.  Returns the device scalar holding
-        sum(err^2) (mse) or sum|err| (l1) of this batch."""
-        if not self.fused_supported():
-            raise NotImplementedError("no fused kernel for this configuration; use field() + autograd")
-        self._ensure_grads()
+    def grad_arena(self, replicas=None):
+        """FP32 arena [replicas, T] holding every table gradient (T = sum of the
+        table sizes, each padded to 64 floats).  The replica-0 views become the
+        tables' ``.grad``; the fused training kernels add into the replica of
+        the XCD they run on.  Returns (arena, T, replicas)."""
+        R = self.grad_replicas if replicas is None else replicas
         b, c = self._tables()
-        d = self._spec.desc(*self._split(self._read_tables()))
-        layers = self._layers()
-        m = mlp_desc(layers)
+        tabs = b + c
+        st = getattr(self, "_arena", None)
+        ok = (st is not None and st[2] == R and len(st[3]) == len(tabs)
+              and all(t.grad is not None and t.grad.data_ptr() == st[0].data_ptr() + 4 * off
+                      for t, off in zip(tabs, st[3])))
+        if not ok:
+            offs, T = [], 0
+            for t in tabs:
+                offs.append(T)
+                T += (t.numel() + 63) // 64 * 64
+            arena = torch.zeros((R, T), device=self.device, dtype=torch.float32)
+            for t, off in zip(tabs, offs):
+                view = torch.as_strided(arena, t.shape, t.stride(), off)
+                if t.grad is not None:
+                    view.copy_(t.grad)
+                t.grad = view
+                # FusedAdam sums the copies only while .grad is still this view
+                t._ff_rep = (R, T, view.data_ptr())
+            st = (arena, T, R, offs)
+            self._arena = st
+        return st[0], st[1], st[2]
+
+    def reduce_grad_replicas(self):
+        """Sum the per-XCD gradient copies into the tables' ``.grad`` (needed
+        before an optimizer other than FusedAdam reads them)."""
+        st = getattr(self, "_arena", None)
+        if st is None or st[2] <= 1:
+            return
+        arena, T, R, _ = st
+        L.check(L.lib().ff_reduce_replicas(L.ptr(arena), R, T, T, L.stream_of(arena)), "ff_reduce_replicas")
+
+    def _train_grads(self):
+        """Grads struct for the fused training kernels (tables in the arena)."""
+        self._ensure_grads()
+        arena, T, R = self.grad_arena()
+        b, c = self._tables()
         p = self.linear_mat.params_list()
         g = L.Grads()
         for l, t in enumerate(b):
@@ -281,6 +314,20 @@ class FactorFields(RenderMixin, nn.Module):
         g.w0 = p[0].grad.data_ptr()
         if len(p) == 3:
             g.b0, g.w1 = p[1].grad.data_ptr(), p[2].grad.data_ptr()
+        g.replicas, g.replica_stride = R, T
+        return g
+
+    def regression_step(self, x, target, loss="mse", loss_scale=1.0, loss_sum=None, reduce=True):
+        """Forward, loss and backward of one regression batch in one kernel
+        (scripts/sdf_regression.ipynb:369-379).  Gradients are accumulated
+        into each parameter's ``.grad``; with reduce=False the table
+        gradients stay split over the per-XCD copies for FusedAdam to sum.
+        Returns the device scalar holding sum(err^2) (mse) or sum|err| (l1)."""
+        if not self.fused_supported():
+            raise NotImplementedError("no fused kernel for this configuration; use field() + autograd")
+        g = self._train_grads()
+        d = self._spec.desc(*self._split(self._read_tables()))
+        m = mlp_desc(self._layers())
         x = x.contiguous()
         target = target.contiguous()
         n = x.shape[0]
@@ -293,6 +340,8 @@ class FactorFields(RenderMixin, nn.Module):
                                                   ctypes.byref(g), L.ptr(loss_sum), L.ptr(ws),
                                                   ws_bytes, L.stream_of(x)),
                 "ff_field_train_regression")
+        if reduce:
+            self.reduce_grad_replicas()
         return loss_sum
 
     def _split(self, tabs):
@@ -326,6 +375,9 @@ class FactorFields(RenderMixin, nn.Module):
             rm = getattr(self, "renderModule", None)
             if rm is not None and len(list(rm.parameters())):
                 groups += [{"params": rm.parameters(), "lr": lr_small}]
+        if getattr(self, "proposal", None) is not None:
+            groups += [{"params": [self.proposal.grid], "lr": lr_large},
+                       {"params": self.proposal.mlp.parameters(), "lr": lr_small}]
         return groups
 
     # ------------------------------------------------ models/FactorFields.py:669-691

@@ -47,9 +47,12 @@ class FusedAdam(torch.optim.Optimizer):
                     st["exp_avg_sq"] = torch.zeros_like(p, memory_format=torch.preserve_format)
                 st["step"] += 1
                 shadow = self.shadow_of(p) if self.shadow_of is not None else None
+                R, stride, where = getattr(p, "_ff_rep", (1, 0, None))
+                if where != g.data_ptr():
+                    R, stride = 1, 0
                 t = L.AdamTensor(p.data_ptr(), g.data_ptr(), st["exp_avg"].data_ptr(),
                                  st["exp_avg_sq"].data_ptr(), 0 if shadow is None else shadow.data_ptr(),
-                                 p.numel(), float(group["lr"]))
+                                 p.numel(), float(group["lr"]), R, stride)
                 key = (st["step"], float(b1), float(b2), float(group["eps"]), p.device)
                 buckets.setdefault(key, []).append(t)
         for (step, b1, b2, eps, dev), ts in buckets.items():

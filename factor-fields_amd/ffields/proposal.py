@@ -1,0 +1,84 @@
+"""Learned proposal sampler: a 64^3 x 8 density grid + MLP 8 -> 32 -> 1,
+trained online on the field's own rendering weights, that places the fine
+samples of each ray (sample_pdf semantics, dataLoader/ray_utils.py:132-174).
+All arithmetic runs in libffields.so (ff_proposal_sample / ff_proposal_bwd).
+"""
+
+import ctypes
+
+import torch
+import torch.nn as nn
+
+from . import _lib as L
+from .ops import texel_major
+
+
+class ProposalNet(nn.Module):
+    def __init__(self, aabb, reso=64, hidden=32, n_fine=48, density_shift=-5.0, distance_scale=25.0,
+                 near=0.05, table_dtype="bf16", device="cuda", seed=0):
+        super().__init__()
+        if hidden != 32:
+            raise NotImplementedError("the proposal kernels are built for 32 hidden units")
+        self.device = torch.device(device)
+        g = torch.Generator().manual_seed(seed)
+        grid = torch.randn((1, 8, reso, reso, reso), generator=g) * 0.1
+        self.grid = nn.Parameter(texel_major(grid.to(self.device)))
+        self.mlp = nn.ModuleList([nn.Linear(8, hidden), nn.Linear(hidden, 1, bias=False)]).to(self.device)
+        self.aabb = torch.as_tensor(aabb, dtype=torch.float32).to(self.device)
+        self.reso = reso
+        self.n_fine = n_fine
+        self.density_shift = float(density_shift)
+        self.distance_scale = float(distance_scale)
+        self.near = float(near)
+        self.table_dtype = table_dtype
+        self._shadow = None
+        if table_dtype == "bf16":
+            self.refresh_shadow()
+
+    def refresh_shadow(self):
+        t = self.grid
+        s = torch.empty(t.shape, dtype=torch.bfloat16, device=t.device, memory_format=torch.channels_last_3d)
+        L.check(L.lib().ff_cast_bf16(L.ptr(t), L.ptr(s), t.numel(), L.stream_of(t)), "ff_cast_bf16")
+        self._shadow = s
+
+    def shadow_of(self, p):
+        return self._shadow if (p is self.grid and self.table_dtype == "bf16") else None
+
+    def desc(self):
+        d = L.ProposalDesc()
+        d.reso[0] = d.reso[1] = d.reso[2] = self.reso
+        d.table_dtype = L.BF16 if self.table_dtype == "bf16" else L.F32
+        d.grid = (self._shadow if self.table_dtype == "bf16" else self.grid).data_ptr()
+        d.w0 = self.mlp[0].weight.data_ptr()
+        d.b0 = self.mlp[0].bias.data_ptr()
+        d.w1 = self.mlp[1].weight.data_ptr()
+        a = self.aabb.cpu().tolist()
+        for k in range(3):
+            d.aabb_min[k], d.aabb_max[k] = a[0][k], a[1][k]
+        d.density_shift = self.density_shift
+        d.distance_scale = self.distance_scale
+        d.near = self.near
+        d.n_fine = self.n_fine
+        return d
+
+    def sample(self, rays, samples_struct, prop_u=None, fine_u=None):
+        d = self.desc()
+        L.check(L.lib().ff_proposal_sample(ctypes.byref(d), L.ptr(rays), L.ptr(prop_u), L.ptr(fine_u),
+                                           rays.shape[0], ctypes.byref(samples_struct), L.stream_of(rays)),
+                "ff_proposal_sample")
+
+    def backward(self, rays, samples_struct, weights, prop_u=None, loss_scale=1.0, loss_sum=None):
+        for p in self.parameters():
+            if p.grad is None:
+                p.grad = torch.zeros_like(p, memory_format=torch.preserve_format)
+        g = L.ProposalGrads()
+        g.grid = self.grid.grad.data_ptr()
+        g.w0 = self.mlp[0].weight.grad.data_ptr()
+        g.b0 = self.mlp[0].bias.grad.data_ptr()
+        g.w1 = self.mlp[1].weight.grad.data_ptr()
+        g.replicas, g.replica_stride = 1, 0
+        d = self.desc()
+        L.check(L.lib().ff_proposal_bwd(ctypes.byref(d), L.ptr(rays), L.ptr(prop_u), rays.shape[0],
+                                        ctypes.byref(samples_struct), L.ptr(weights), float(loss_scale),
+                                        ctypes.byref(g), L.ptr(loss_sum), L.stream_of(rays)),
+                "ff_proposal_bwd")

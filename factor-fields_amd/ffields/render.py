@@ -124,16 +124,21 @@ class RenderMixin:
         return d, m
 
     def render_buffers(self, B, Ns):
+        """Packed-sample and per-sample buffers for B rays x Ns candidates,
+        kept between steps (a few shapes are cached)."""
         key = (B, Ns)
-        cache = getattr(self, "_rbuf", None)
-        if cache is None or cache[0] != key:
+        cache = self.__dict__.setdefault("_rbufs", {})
+        if key not in cache:
+            if len(cache) >= 4:
+                cache.pop(next(iter(cache)))
             dev = self.device
             buf = alloc_samples(B, Ns, dev)
             cap = buf["xyz"].shape[0]
             buf.update(sigma=torch.empty(cap, device=dev), rgb=torch.empty((cap, 3), device=dev),
                        d_sample=torch.empty((cap, 4), device=dev))
-            self._rbuf = (key, buf)
-        return self._rbuf[1]
+            cache[key] = buf
+        self._rbuf = (key, cache[key])
+        return cache[key]
 
     def render_samples(self, rays, n_samples, jitter, buf):
         """Sampler + field + head over `rays`; fills buf (packed samples,
@@ -150,17 +155,6 @@ class RenderMixin:
                                             L.ptr(buf["rgb"]), st), "ff_render_field_fwd")
         return s
 
-    def _grads_struct(self):
-        self._ensure_grads()
-        b, c = self._tables()
-        p = self.linear_mat.params_list()
-        g = L.Grads()
-        for l, t in enumerate(b):
-            g.basis[l] = t.grad.data_ptr()
-        for k, t in enumerate(c):
-            g.coeff[k] = t.grad.data_ptr()
-        g.w0, g.b0, g.w1 = p[0].grad.data_ptr(), p[1].grad.data_ptr(), p[2].grad.data_ptr()
-        return g
 
     def _jitter(self, rays, is_train, jitter):
         if jitter is not None:
@@ -183,11 +177,123 @@ class RenderMixin:
         coeffs = torch.zeros((1, sum(self.basis_dims)), device=rays.device)
         return rgb_map, depth, coeffs
 
+    # ------------------------------------------------ learned proposal sampler
+    def enable_proposal(self, reso=64, n_fine=48, table_dtype="bf16", density_shift=-5.0, seed=0):
+        """Attach a ProposalNet (64^3 x 8 grid + MLP 8-32-1) that places n_fine
+        samples per ray; the occupancy sampler stays available as fallback
+        and judge (use_proposal toggles which one trains)."""
+        from .proposal import ProposalNet
+        self.proposal = ProposalNet(self.aabb.cpu(), reso=reso, n_fine=n_fine, density_shift=density_shift,
+                                    distance_scale=float(self.cfg.renderer.distance_scale),
+                                    table_dtype=table_dtype, device=self.device, seed=seed)
+        self.use_proposal = True
+        return self.proposal
+
+    def shadow_lookup(self, p):
+        s = self.shadow_of(p)
+        if s is None and getattr(self, "proposal", None) is not None:
+            s = self.proposal.shadow_of(p)
+        return s
+
+    def _composite(self, s, B, buf, bg, rgb_map, rgb_raw, depth=None, weights=None, stream=None):
+        thres = float(self.cfg.renderer.rayMarch_weight_thres)
+        L.check(L.lib().ff_composite_fwd(ctypes.byref(s), B, L.ptr(buf["sigma"]), L.ptr(buf["rgb"]), int(bg),
+                                         thres, L.ptr(rgb_map), L.ptr(rgb_raw), L.ptr(depth), None,
+                                         L.ptr(weights), stream), "ff_composite_fwd")
+
+    def _field_fwd(self, rays, s, buf, stream):
+        d, m = self._field_structs()
+        h = self._head()
+        L.check(L.lib().ff_render_field_fwd(ctypes.byref(d), ctypes.byref(m), ctypes.byref(h), L.ptr(rays),
+                                            rays.shape[0], ctypes.byref(s), L.ptr(buf["sigma"]),
+                                            L.ptr(buf["rgb"]), stream), "ff_render_field_fwd")
+
+    def _train_from_samples(self, rays, target, s, buf, bg, loss_scale, loss_sum, weights=None):
+        """composite forward (+weights) + MSE backward + field backward."""
+        B = rays.shape[0]
+        st = L.stream_of(rays)
+        rgb_map = _buf(buf, "rgb_map", (B, 3), rays.device)
+        rgb_raw = _buf(buf, "rgb_raw", (B, 3), rays.device)
+        thres = float(self.cfg.renderer.rayMarch_weight_thres)
+        self._composite(s, B, buf, bg, rgb_map, rgb_raw, weights=weights, stream=st)
+        L.check(L.lib().ff_composite_bwd(ctypes.byref(s), B, L.ptr(buf["sigma"]), L.ptr(buf["rgb"]),
+                                         int(bg), thres, L.ptr(rgb_raw), None, L.ptr(target),
+                                         float(loss_scale), L.ptr(loss_sum), L.ptr(buf["d_sample"]), st),
+                "ff_composite_bwd")
+        d, m = self._field_structs()
+        h = self._head()
+        g = self._train_grads()
+        ws_bytes = L.lib().ff_render_field_bwd_workspace_bytes(ctypes.byref(d), ctypes.byref(m), s.cap)
+        ws = self._workspace(ws_bytes)
+        L.check(L.lib().ff_render_field_bwd(ctypes.byref(d), ctypes.byref(m), ctypes.byref(h), L.ptr(rays),
+                                            B, ctypes.byref(s), L.ptr(buf["d_sample"]), ctypes.byref(g),
+                                            L.ptr(ws), ws_bytes, st), "ff_render_field_bwd")
+        return rgb_map
+
+    def render_train_step_proposal(self, rays, target, white_bg=True, loss_scale=1.0, prop_loss_scale=1.0,
+                                   loss_sum=None, prop_loss_sum=None, reduce=True, prop_u=None, fine_u=None):
+        """NeRF training pass with the learned proposal placing the samples:
+        proposal sample -> field -> composite (weights kept) -> MSE backward ->
+        field backward -> proposal backward on the field's weights."""
+        pr = self.proposal
+        rays = rays.contiguous()
+        target = target.contiguous()
+        B = rays.shape[0]
+        nf = pr.n_fine
+        if prop_u is None:
+            prop_u = torch.rand((B, 64), device=rays.device)
+        if fine_u is None:
+            fine_u = torch.rand((B, nf), device=rays.device)
+        bg = bool(white_bg or torch.rand((1,)) < 0.5)
+        buf = self.render_buffers(B, nf)
+        s = _samples_struct(buf)
+        st = L.stream_of(rays)
+        pr.sample(rays, s, prop_u, fine_u)
+        self._field_fwd(rays, s, buf, st)
+        if loss_sum is None:
+            loss_sum = torch.zeros((), device=rays.device)
+        weights = _buf(buf, "weights", (s.cap,), rays.device)
+        self._train_from_samples(rays, target, s, buf, bg, loss_scale, loss_sum, weights=weights)
+        pr.backward(rays, s, weights, prop_u=prop_u, loss_scale=prop_loss_scale, loss_sum=prop_loss_sum)
+        if reduce:
+            self.reduce_grad_replicas()
+        return loss_sum
+
+    @torch.no_grad()
+    def render_eval(self, rays, white_bg=True, use_proposal=None, N_samples=-1):
+        """Forward-only render of `rays` with the occupancy sampler or the
+        proposal (deterministic draws).  Returns rgb_map [B, 3]."""
+        use_proposal = getattr(self, "use_proposal", False) if use_proposal is None else use_proposal
+        rays = rays.contiguous()
+        B = rays.shape[0]
+        st = L.stream_of(rays)
+        if use_proposal:
+            pr = self.proposal
+            buf = alloc_samples(B, pr.n_fine, rays.device)
+            s = _samples_struct(buf)
+            pr.sample(rays, s)
+        else:
+            n_samples = N_samples if N_samples > 0 else self.nSamples
+            buf = alloc_samples(B, n_samples, rays.device)
+            s = _samples_struct(buf)
+            smp = self._sampler(n_samples)
+            L.check(L.lib().ff_sample_rays(ctypes.byref(smp), L.ptr(rays), None, B, ctypes.byref(s), st),
+                    "ff_sample_rays")
+        cap = buf["xyz"].shape[0]
+        buf.update(sigma=torch.empty(cap, device=rays.device), rgb=torch.empty((cap, 3), device=rays.device))
+        self._field_fwd(rays, s, buf, st)
+        rgb_map = torch.empty((B, 3), device=rays.device)
+        rgb_raw = torch.empty((B, 3), device=rays.device)
+        self._composite(s, B, buf, white_bg, rgb_map, rgb_raw, stream=st)
+        return rgb_map
+
     def render_train_step(self, rays, target, white_bg=True, N_samples=-1, jitter=None, loss_scale=1.0,
-                          loss_sum=None):
+                          loss_sum=None, reduce=True):
         """One NeRF training pass fused across kernels: sample, field, composite,
         MSE loss (train_per_scene.py:158), composite backward, field backward
-        into each parameter's .grad.  Returns the device scalar sum((rgb-gt)^2)."""
+        into each parameter's .grad (reduce=False leaves the table gradients
+        split over the per-XCD copies for FusedAdam).  Returns the device
+        scalar sum((rgb-gt)^2)."""
         n_samples = N_samples if N_samples > 0 else self.nSamples
         rays = rays.contiguous()
         target = target.contiguous()
@@ -196,32 +302,136 @@ class RenderMixin:
         bg = bool(white_bg or torch.rand((1,)) < 0.5)
         buf = self.render_buffers(B, n_samples)
         s = self.render_samples(rays, n_samples, jit, buf)
-        st = L.stream_of(rays)
-        rgb_map = buf.setdefault("rgb_map", torch.empty((B, 3), device=rays.device))
-        if rgb_map.shape[0] != B:
-            rgb_map = buf["rgb_map"] = torch.empty((B, 3), device=rays.device)
-        rgb_raw = buf.setdefault("rgb_raw", torch.empty((B, 3), device=rays.device))
-        if rgb_raw.shape[0] != B:
-            rgb_raw = buf["rgb_raw"] = torch.empty((B, 3), device=rays.device)
-        thres = float(self.cfg.renderer.rayMarch_weight_thres)
-        L.check(L.lib().ff_composite_fwd(ctypes.byref(s), B, L.ptr(buf["sigma"]), L.ptr(buf["rgb"]),
-                                         int(bg), thres, L.ptr(rgb_map), L.ptr(rgb_raw), None, None, None,
-                                         st), "ff_composite_fwd")
         if loss_sum is None:
             loss_sum = torch.zeros((), device=rays.device)
-        L.check(L.lib().ff_composite_bwd(ctypes.byref(s), B, L.ptr(buf["sigma"]), L.ptr(buf["rgb"]),
-                                         int(bg), thres, L.ptr(rgb_raw), None, L.ptr(target),
-                                         float(loss_scale), L.ptr(loss_sum), L.ptr(buf["d_sample"]), st),
-                "ff_composite_bwd")
+        self._train_from_samples(rays, target, s, buf, bg, loss_scale, loss_sum)
+        if reduce:
+            self.reduce_grad_replicas()
+        return loss_sum
+
+
+    # ------------------------------------------------ occupancy maintenance
+    @torch.no_grad()
+    def getDenseAlpha(self, gridSize=None, times=16, generator=None):
+        """models/FactorFields.py:729-755: mean over `times` jittered passes of
+        1 - exp(-sigma * stepSize * distance_scale) on a dense grid.  Returns
+        (alpha [Z, Y, X], dense_xyz [Z, Y, X, 3])."""
+        gridSize = self.gridSize.tolist() if gridSize is None else list(gridSize)
+        dev = self.device
+        aabb = self.inward_aabb
+        aabbSize = aabb[1] - aabb[0]
+        units = aabbSize / (torch.LongTensor(gridSize).to(dev) - 1)
+        units_half = 1.0 / (torch.LongTensor(gridSize) - 1) * 0.5
+        stepSize = torch.mean(units)
+        samples = torch.stack(torch.meshgrid(
+            [torch.linspace(units_half[0], 1 - units_half[0], gridSize[0]),
+             torch.linspace(units_half[1], 1 - units_half[1], gridSize[1]),
+             torch.linspace(units_half[2], 1 - units_half[2], gridSize[2])], indexing="ij"), -1).to(dev)
+        dense_xyz = aabb[0] * (1 - samples) + aabb[1] * samples
+        dense_xyz = dense_xyz.transpose(0, 2).contiguous()
+        alpha = torch.zeros(dense_xyz.shape[:-1], device=dev)
+        length = float(stepSize * self.cfg.renderer.distance_scale)
         d, m = self._field_structs()
         h = self._head()
-        g = self._grads_struct()
-        ws_bytes = L.lib().ff_render_field_bwd_workspace_bytes(ctypes.byref(d), ctypes.byref(m), s.cap)
-        ws = self._workspace(ws_bytes)
-        L.check(L.lib().ff_render_field_bwd(ctypes.byref(d), ctypes.byref(m), ctypes.byref(h), L.ptr(rays),
-                                            B, ctypes.byref(s), L.ptr(buf["d_sample"]), ctypes.byref(g),
-                                            L.ptr(ws), ws_bytes, st), "ff_render_field_bwd")
-        return loss_sum
+        occ = self._sampler(1)
+        flat = dense_xyz.view(-1, 3)
+        for _ in range(times):
+            if times > 1:
+                shift = (torch.rand(flat.shape, device=dev, generator=generator) * 2 - 1) * (units / 2 * 1.2)
+                pts = (flat + shift).contiguous()
+            else:
+                pts = flat
+            L.check(L.lib().ff_render_alpha(ctypes.byref(d), ctypes.byref(m), ctypes.byref(h),
+                                            ctypes.byref(occ), L.ptr(pts), pts.shape[0], length,
+                                            L.ptr(alpha), L.stream_of(pts)), "ff_render_alpha")
+        return alpha / times, dense_xyz
+
+    @torch.no_grad()
+    def updateAlphaMask(self, gridSize=(200, 200, 200), is_update_alphaMask=False, times=16, generator=None):
+        """models/FactorFields.py:757-793.  Returns the new aabb."""
+        gridSize = list(gridSize)
+        alpha, dense_xyz = self.getDenseAlpha(gridSize, times=times, generator=generator)
+        thres = float(self.cfg.renderer.alphaMask_thres) if is_update_alphaMask else 0.08
+        Z, Y, X = alpha.shape
+        occ = torch.empty((Z, Y, X), dtype=torch.uint8, device=self.device)
+        L.check(L.lib().ff_occupancy_pool(L.ptr(alpha), Z, Y, X, 1.0, thres, L.ptr(occ),
+                                          L.stream_of(alpha)), "ff_occupancy_pool")
+        keep = _remove_small_objects(occ.cpu().numpy().astype(bool), min_size=float(np.mean([Z, Y, X])))
+        alpha = torch.from_numpy(keep.astype(np.float32)).to(self.device)
+        if is_update_alphaMask:
+            self.alphaMask = AlphaGridMask(self.device, self.inward_aabb, alpha)
+        valid_xyz = dense_xyz[alpha > 0.5]
+        xyz_min = valid_xyz.amin(0)
+        xyz_max = valid_xyz.amax(0)
+        pad = (xyz_max - xyz_min) / 20
+        return torch.stack((xyz_min - pad, xyz_max + pad))
+
+    @torch.no_grad()
+    def shrink(self, new_aabb):
+        """models/FactorFields.py:796-809 (re-initialises every table, as the
+        reference does)."""
+        self.setup_params(new_aabb.tolist())
+        if self.coeff_type != "none":
+            del self.coeffs
+            self.coeffs = self.init_coef()
+        if self.basis_type != "none":
+            del self.basises
+            self.basises = self.init_basis()
+        self.aabb = self.inward_aabb = new_aabb.to(self.device)
+        self.cfg.dataset.aabb = self.aabb.tolist()
+        if self.table_dtype == "bf16":
+            self.refresh_shadows()
+        self._build_spec()
+        self.update_renderParams(self.gridSize.tolist())
+
+    @torch.no_grad()
+    def filtering_rays(self, all_rays, all_rgbs, N_samples=256, chunk=10240 * 5, bbox_only=False):
+        """models/FactorFields.py:811-841: keep rays that hit the box
+        (bbox_only) or reach an occupied cell (> 0 in the occupancy volume)."""
+        keep = []
+        for k in range(0, all_rays.shape[0], chunk):
+            rc = all_rays[k:k + chunk].to(self.device).contiguous()
+            if bbox_only or self.alphaMask is None:
+                o, d = rc[:, :3], rc[:, 3:6]
+                vec = torch.where(d == 0, torch.full_like(d, 1e-6), d)
+                ra = (self.aabb[1] - o) / vec
+                rb = (self.aabb[0] - o) / vec
+                t_min = torch.minimum(ra, rb).amax(-1)
+                t_max = torch.maximum(ra, rb).amin(-1)
+                keep.append((t_max > t_min).cpu())
+            else:
+                buf = alloc_samples(rc.shape[0], N_samples, self.device)
+                s = _samples_struct(buf)
+                smp = self._sampler(N_samples)
+                L.check(L.lib().ff_sample_rays(ctypes.byref(smp), L.ptr(rc), None, rc.shape[0],
+                                               ctypes.byref(s), L.stream_of(rc)), "ff_sample_rays")
+                cnt = buf["ray_off"][1:] - buf["ray_off"][:-1]
+                keep.append((cnt > 0).cpu())
+        mask = torch.cat(keep)
+        return all_rays[mask], all_rgbs[mask]
+
+
+def _buf(buf, key, shape, device):
+    t = buf.get(key)
+    if t is None or tuple(t.shape) != tuple(shape):
+        t = buf[key] = torch.empty(shape, device=device)
+    return t
+
+
+def _remove_small_objects(mask, min_size):
+    """skimage.morphology.remove_small_objects(mask, min_size, connectivity=1)
+    (models/FactorFields.py:774-775): drop face-connected components with fewer
+    than min_size voxels."""
+    from scipy import ndimage
+    lab, n = ndimage.label(mask, structure=ndimage.generate_binary_structure(mask.ndim, 1))
+    if n == 0:
+        return mask
+    sizes = np.bincount(lab.ravel())
+    small = sizes < min_size
+    small[0] = False
+    out = mask.copy()
+    out[small[lab]] = False
+    return out
 
 
 class RenderRays(torch.autograd.Function):

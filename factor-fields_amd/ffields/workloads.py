@@ -158,3 +158,133 @@ class SdfScene:
         surf = self.surface_points(n - nu)
         pts = torch.cat([uni, surf])
         return pts.float(), self.sdf(pts).float()[:, None]
+
+
+# ---------------------------------------------------------------- NeRF scene
+class NerfScene:
+    """Density = sum of 48 seeded anisotropic Gaussians (centres U[-1,1]^3,
+    per-axis std U[0.05,0.4], random rotation, peak sigma U[10,100]) inside
+    [-1.5,1.5]^3; RGB = 0.5 + 0.5*sin(2*pi*f_c.x + phi_c) per channel with
+    |f_c| U[0.1,0.5] cycles/unit.  Cameras: `n_views` pinhole views of HxW,
+    FoV 40 deg, on a radius-4 Fibonacci sphere looking at the origin.  Ground
+    truth: 1,024 stratified (mid-point) samples per ray over the ray/box
+    interval, white background."""
+
+    AABB = ((-1.5, -1.5, -1.5), (1.5, 1.5, 1.5))
+
+    def __init__(self, seed=BASE_SEED + 2000, n_gauss=48, n_views=100, H=800, W=800, fov_deg=40.0,
+                 radius=4.0, device="cuda"):
+        g = torch.Generator().manual_seed(seed)
+        d = torch.float64
+        self.device = torch.device(device)
+        self.mu = torch.rand((n_gauss, 3), generator=g, dtype=d) * 2 - 1
+        std = 0.05 + torch.rand((n_gauss, 3), generator=g, dtype=d) * 0.35
+        q, r = torch.linalg.qr(torch.randn((n_gauss, 3, 3), generator=g, dtype=d))
+        q = q * torch.sign(torch.diagonal(r, dim1=-2, dim2=-1))[:, None, :]
+        self.prec = q @ torch.diag_embed(1.0 / std ** 2) @ q.transpose(-1, -2)
+        self.peak = 10 + torch.rand(n_gauss, generator=g, dtype=d) * 90
+        fdir = torch.randn((3, 3), generator=g, dtype=d)
+        fdir = fdir / fdir.norm(dim=-1, keepdim=True)
+        self.freq = fdir * (0.1 + torch.rand((3, 1), generator=g, dtype=d) * 0.4)
+        self.phase = torch.rand(3, generator=g, dtype=d) * 2 * math.pi
+        self.H, self.W = H, W
+        self.focal = 0.5 * W / math.tan(0.5 * math.radians(fov_deg))
+        self.c2w = self._cameras(n_views, radius)
+        self._dev = {k: v.to(self.device, torch.float32) for k, v in
+                     dict(mu=self.mu, prec=self.prec, peak=self.peak, freq=self.freq,
+                          phase=self.phase).items()}
+
+    @staticmethod
+    def _cameras(n, radius):
+        ga = math.pi * (3 - math.sqrt(5))
+        out = []
+        for i in range(n):
+            z = 1 - 2 * (i + 0.5) / n
+            rr = math.sqrt(max(0.0, 1 - z * z))
+            p = torch.tensor([rr * math.cos(ga * i), rr * math.sin(ga * i), z], dtype=torch.float64) * radius
+            fwd = -p / p.norm()
+            up = torch.tensor([0.0, 0.0, 1.0], dtype=torch.float64)
+            if abs(float(fwd @ up)) > 0.99:
+                up = torch.tensor([0.0, 1.0, 0.0], dtype=torch.float64)
+            right = torch.linalg.cross(fwd, up)
+            right = right / right.norm()
+            up2 = torch.linalg.cross(right, fwd)
+            # camera looks along -z (OpenGL / Blender convention)
+            out.append(torch.stack([right, up2, -fwd, p], -1))
+        return torch.stack(out)                         # [n, 3, 4]
+
+    def rays(self, view, pix):
+        """Rays [len(pix), 6] of pixel indices `pix` (row-major) of one view
+        (directions through pixel centres, normalised)."""
+        c2w = self.c2w[view].to(self.device, torch.float32)
+        j = (pix // self.W).float() + 0.5
+        i = (pix % self.W).float() + 0.5
+        dirs = torch.stack([(i - self.W / 2) / self.focal, -(j - self.H / 2) / self.focal,
+                            -torch.ones_like(i)], -1)
+        d = dirs @ c2w[:, :3].T
+        d = d / d.norm(dim=-1, keepdim=True)
+        o = c2w[:, 3].expand_as(d)
+        return torch.cat([o, d], -1)
+
+    def random_rays(self, n, generator):
+        view = torch.randint(0, len(self.c2w), (n,), device=self.device, generator=generator)
+        pix = torch.randint(0, self.H * self.W, (n,), device=self.device, generator=generator)
+        out = torch.empty((n, 6), device=self.device)
+        for v in range(len(self.c2w)):
+            sel = view == v
+            if bool(sel.any()):
+                out[sel] = self.rays(v, pix[sel])
+        return out
+
+    def _quad_coeffs(self):
+        """q_g(x) = (x-mu_g)^T P_g (x-mu_g) as coefficients of the quadratic
+        features [x^2, y^2, z^2, xy, xz, yz, x, y, z, 1] (float64 -> float32)."""
+        if getattr(self, "_A", None) is None:
+            P, mu = self.prec, self.mu
+            Pm = (P @ mu[:, :, None])[..., 0]
+            A = torch.stack([P[:, 0, 0], P[:, 1, 1], P[:, 2, 2], 2 * P[:, 0, 1], 2 * P[:, 0, 2],
+                             2 * P[:, 1, 2], -2 * Pm[:, 0], -2 * Pm[:, 1], -2 * Pm[:, 2],
+                             (mu * Pm).sum(-1)], -1)
+            self._A = A.to(self.device, torch.float32)
+        return self._A
+
+    def density(self, x):
+        A = self._quad_coeffs()
+        x0, x1, x2 = x[:, 0], x[:, 1], x[:, 2]
+        F = torch.stack([x0 * x0, x1 * x1, x2 * x2, x0 * x1, x0 * x2, x1 * x2, x0, x1, x2,
+                         torch.ones_like(x0)], -1)
+        q = (F @ A.T).clamp(min=0)
+        return torch.exp(-0.5 * q) @ self._dev["peak"]
+
+    def color(self, x):
+        p = self._dev
+        return 0.5 + 0.5 * torch.sin(2 * math.pi * (x @ p["freq"].T) + p["phase"])
+
+    @torch.no_grad()
+    def render_gt(self, rays, n_samples=1024, chunk=4096, white_bg=True):
+        out = []
+        lo = torch.tensor(self.AABB[0], device=self.device)
+        hi = torch.tensor(self.AABB[1], device=self.device)
+        for k in range(0, rays.shape[0], chunk):
+            r = rays[k:k + chunk]
+            o, d = r[:, :3], r[:, 3:6]
+            vec = torch.where(d == 0, torch.full_like(d, 1e-6), d)
+            ta, tb = (hi - o) / vec, (lo - o) / vec
+            t0 = torch.minimum(ta, tb).amax(-1).clamp(min=0.0)
+            t1 = torch.maximum(ta, tb).amin(-1)
+            hit = t1 > t0
+            t1 = torch.where(hit, t1, t0)
+            u = (torch.arange(n_samples, device=self.device).float() + 0.5) / n_samples
+            t = t0[:, None] + (t1 - t0)[:, None] * u
+            delta = ((t1 - t0) / n_samples)[:, None].expand_as(t)
+            x = (o[:, None, :] + d[:, None, :] * t[..., None]).reshape(-1, 3)
+            sig = self.density(x).view(t.shape)
+            col = self.color(x).view(*t.shape, 3)
+            alpha = 1 - torch.exp(-sig * delta)
+            T = torch.cumprod(torch.cat([torch.ones_like(alpha[:, :1]), 1 - alpha], -1), -1)
+            w = alpha * T[:, :-1]
+            c = (w[..., None] * col).sum(1)
+            if white_bg:
+                c = c + T[:, -1:]
+            out.append(c.clamp(0, 1))
+        return torch.cat(out)

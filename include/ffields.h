@@ -94,13 +94,21 @@ typedef struct ff_mlp_desc {
   const float* w1;
 } ff_mlp_desc;
 
-/* FP32 gradient accumulators matching ff_field_desc / ff_mlp_desc. */
+/* FP32 gradient accumulators matching ff_field_desc / ff_mlp_desc.
+ * replicas > 1: every table gradient pointer is the first of `replicas`
+ * copies spaced `replica_stride` floats apart (one flat arena), and each wave
+ * of a scatter kernel adds into the copy of the XCD it runs on, so no cache
+ * line is shared between the XCDs' L2 caches.  The copies are summed by
+ * ff_reduce_replicas or by ff_adam_multi.  The decoder gradients w0/b0/w1
+ * are never replicated. */
 typedef struct ff_grads {
   float* basis[FF_MAX_LEVELS];
   float* coeff[6];
   float* w0;
   float* b0;
   float* w1;
+  int32_t replicas;
+  int64_t replica_stride;
 } ff_grads;
 
 const char* ff_status_str(int status);
@@ -184,9 +192,14 @@ typedef struct ff_adam_tensor {
   uint16_t* shadow_bf16; /* may be NULL */
   int64_t n;
   float lr;
+  int32_t grad_replicas;  /* > 1: grad is the first of this many copies (summed here) */
+  int64_t grad_stride;    /* floats between copies */
 } ff_adam_tensor;
 int ff_adam_multi(const ff_adam_tensor* tensors, int count, float beta1, float beta2, float eps,
                   int64_t step, int zero_grad, void* stream);
+
+/* arena[i] = sum_r arena[r * stride + i] for i < n, copies 1.. cleared. */
+int ff_reduce_replicas(float* arena, int32_t replicas, int64_t n, int64_t stride, void* stream);
 
 /* FP32 -> BF16 (round to nearest even) copy, used to refresh BF16 table shadows. */
 int ff_cast_bf16(const float* src, uint16_t* dst, int64_t n, void* stream);
@@ -266,6 +279,72 @@ int ff_composite_bwd(const ff_samples* s, int64_t n_rays, const float* sigma, co
                      int white_bg, float weight_thres, const float* rgb_raw,
                      const float* d_rgb_map, const float* target, float loss_scale,
                      float* loss_sum, float* d_sample, void* stream);
+
+/* Occupancy-grid maintenance (models/FactorFields.py:710-793).
+ * ff_render_alpha: compute_alpha :710-727 over n points: alpha_accum[i] +=
+ * 1 - exp(-sigma(x_i) * length), where sigma is 0 for points the current
+ * occupancy volume (sampler->occ, tested with > 0 as the reference does; may
+ * be NULL) marks empty.  Only the occupancy fields of `sampler` are used. */
+int ff_render_alpha(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
+                    const ff_ray_sampler* sampler, const float* xyz, int64_t n, float length,
+                    float* alpha_accum, void* stream);
+/* max_pool3d(kernel 3, stride 1, padding 1) of alpha [D, H, W] followed by
+ * `>= thres` (updateAlphaMask :764-772 before the small-object filter):
+ * out [D, H, W] uint8. alpha is divided by `times` and clamped to [0, 1]
+ * first. */
+int ff_occupancy_pool(const float* alpha, int32_t D, int32_t H, int32_t W, float times, float thres,
+                      uint8_t* out, void* stream);
+
+/* ========================================================= proposal path ==
+ * Learned proposal sampler (no counterpart in the reference's training loop;
+ * its resampling follows dataLoader/ray_utils.py:132-174 sample_pdf).  A
+ * low-resolution density grid [Dz, Hy, Wx, 8] (trilinear, align_corners=False,
+ * border) feeds an 8 -> 32 -> 1 MLP (ReLU, softplus(out + shift)); n_prop = 64
+ * stratified samples over the ray/box interval are composited into weights,
+ * and sample_pdf places n_fine samples along the ray.  One wavefront per ray,
+ * one lane per proposal sample. */
+typedef struct ff_proposal_desc {
+  int32_t reso[3];        /* grid extents x, y, z */
+  int32_t table_dtype;    /* FF_F32 or FF_BF16 */
+  const void* grid;       /* [Dz, Hy, Wx, 8] texel-major */
+  const float* w0;        /* [32, 8] */
+  const float* b0;        /* [32] */
+  const float* w1;        /* [1, 32] */
+  float aabb_min[3];
+  float aabb_max[3];
+  float density_shift;
+  float distance_scale;
+  float near;             /* smallest t (sample_point's 0.05) */
+  int32_t n_fine;         /* fine samples per ray, <= 64 */
+} ff_proposal_desc;
+
+/* Fine samples of every ray into `out` (ray-major; n_fine per ray that meets
+ * the box, 0 otherwise).  prop_u [n_rays, 64] jitters the proposal samples in
+ * their bins and fine_u [n_rays, n_fine] the sample_pdf draws (stratified:
+ * u_k = (k + fine_u)/n_fine); NULL means the bin midpoints and
+ * linspace(0, 1, n_fine) (sample_pdf det=True). */
+int ff_proposal_sample(const ff_proposal_desc* p, const float* rays, const float* prop_u,
+                       const float* fine_u, int64_t n_rays, const ff_samples* out, void* stream);
+
+/* Backward of the proposal: interlevel bound loss between the proposal
+ * weights and the field's fine weights (weights [cap] from ff_composite_fwd,
+ * treated as constants): for each fine interval [t_k, t_k+1], bound_k = sum of
+ * the proposal weights of the bins it overlaps and
+ * loss_k = max(0, w_k - bound_k)^2 / (w_k + 1e-7), averaged over rays and
+ * scaled by loss_scale.  Accumulates grid / MLP gradients (grid_grad with
+ * `replicas` per-XCD copies spaced `replica_stride` floats; w0/b0/w1 grads
+ * FP32) and loss_sum. */
+typedef struct ff_proposal_grads {
+  float* grid;
+  float* w0;
+  float* b0;
+  float* w1;
+  int32_t replicas;
+  int64_t replica_stride;
+} ff_proposal_grads;
+int ff_proposal_bwd(const ff_proposal_desc* p, const float* rays, const float* prop_u,
+                    int64_t n_rays, const ff_samples* fine, const float* weights, float loss_scale,
+                    const ff_proposal_grads* grads, float* loss_sum, void* stream);
 
 #ifdef __cplusplus
 }

@@ -179,3 +179,24 @@ def test_adam_matches_torch():
         ref.step()
     assert_close(p1, q1, 1e-6, "p1", atol_frac=1e-7)
     assert_close(p2, q2, 1e-6, "p2", atol_frac=1e-7)
+
+
+def test_grad_replicas_summed_by_adam():
+    """Table gradients split over per-XCD copies: FusedAdam summing them in its
+    pass equals reducing first, and both equal the single-copy result."""
+    from ffields import FusedAdam
+    cfg = small_cfg(**dict(CASES["3d_vm"], out_dim=1))
+    x = rand_points(new_model(cfg), 5000, seed=21).cuda()
+    tgt = torch.randn((5000, 1), generator=torch.Generator().manual_seed(22)).cuda()
+    params = []
+    for reps, reduce in ((1, True), (8, True), (8, False)):
+        m = new_model(cfg, seed=3)
+        m.grad_replicas = reps
+        opt = FusedAdam(m.get_optparam_groups(0.01, 0.02))
+        for _ in range(3):
+            m.regression_step(x, tgt, loss="l1", reduce=reduce)
+            opt.step(zero_grad=True)
+        params.append([p.detach().clone() for p in m.parameters()])
+    for a, b, c in zip(*params):
+        assert_close(b, a, 1e-5, "reduce-first vs single copy", atol_frac=1e-6)
+        assert_close(c, a, 1e-5, "adam-summed vs single copy", atol_frac=1e-6)

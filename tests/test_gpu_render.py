@@ -33,7 +33,8 @@ def nerf_small_cfg(aabb=None):
                    with_dropout=False),
         dataset=dict(aabb=aabb, scene_reso=1024, is_unbound=False),
         renderer=dict(fea2denseAct="softplus", density_shift=-10, distance_scale=25.0, step_ratio=0.5,
-                      max_samples=512, rayMarch_weight_thres=1e-3, shadingMode="SH"),
+                      max_samples=512, rayMarch_weight_thres=1e-3, alphaMask_thres=0.02,
+                      shadingMode="SH"),
         training=dict(volume_resoInit=40, volume_resoFinal=40)))
 
 
@@ -53,8 +54,13 @@ def make_model(seed=0, occupancy=True):
     m = FactorFields(nerf_small_cfg(), "cuda")
     randomize(m, seed, scale=0.5)
     with torch.no_grad():
-        m.linear_mat.backbone[0].bias.add_(3.0)       # some density after the -10 shift
-        m.linear_mat.backbone[1].weight[0].mul_(4.0)
+        m.linear_mat.backbone[0].weight.mul_(20.0)    # density varies in space
+    # centre the density shift so that about 40% of the box is occupied
+    of = oracle_of(m)
+    g = torch.Generator().manual_seed(seed + 7)
+    a = m.aabb.cpu()
+    o0 = of.field(a[0] + (a[1] - a[0]) * torch.rand((4096, 3), generator=g))[:, 0]
+    m.cfg.renderer.density_shift = float(-torch.quantile(o0, 0.6))
     if occupancy:
         g = torch.Generator().manual_seed(seed + 100)
         vol = (torch.rand((29, 41, 37), generator=g) < 0.6).float()
@@ -131,11 +137,12 @@ def test_render_forward_matches_oracle(white_bg):
     B = 512
     rays = make_rays(B, seed=7)
     jit = torch.rand(B, generator=torch.Generator().manual_seed(8))
-    rgb, depth, _ = m(rays.cuda(), white_bg=white_bg, is_train=True, jitter=jit.cuda())
+    rgb, depth, _ = m(rays.cuda(), white_bg=white_bg, is_train=False, jitter=jit.cuda())
     ref_rgb, ref_depth, aux = orr.forward(rays, jitter=jit, white_bg=white_bg)
+    assert float(aux["acc"].max()) > 0.5              # the test scene is not empty
+    assert float(aux["acc"].min()) < 0.5
     assert_close(rgb, ref_rgb, 1e-5, "rgb_map")
     assert_close(depth, ref_depth, 1e-5, "depth")
-    assert float(aux["acc"].max()) > 0.5              # the test scene is not empty
 
 
 @pytest.mark.parametrize("white_bg", [True, False])
@@ -147,7 +154,7 @@ def test_render_backward_matches_oracle(white_bg):
     rays = make_rays(B, seed=9)
     jit = torch.rand(B, generator=torch.Generator().manual_seed(10))
     gt = torch.rand((B, 3), generator=torch.Generator().manual_seed(11))
-    rgb, _, _ = m(rays.cuda(), white_bg=white_bg, is_train=True, jitter=jit.cuda())
+    rgb, _, _ = m(rays.cuda(), white_bg=white_bg, is_train=False, jitter=jit.cuda())
     ((rgb - gt.cuda()) ** 2).mean().backward()
     ref_rgb, _, _ = orr.forward(rays, jitter=jit, white_bg=white_bg)
     ((ref_rgb - gt) ** 2).mean().backward()
@@ -166,7 +173,7 @@ def test_fused_train_step_matches_autograd():
     rays = make_rays(B, seed=12).cuda()
     jit = torch.rand(B, generator=torch.Generator().manual_seed(13)).cuda()
     gt = torch.rand((B, 3), generator=torch.Generator().manual_seed(14)).cuda()
-    rgb, _, _ = m(rays, white_bg=True, is_train=True, jitter=jit)
+    rgb, _, _ = m(rays, white_bg=True, is_train=False, jitter=jit)
     loss = ((rgb - gt) ** 2).mean()
     loss.backward()
     ref = {n: p.grad.clone() for n, p in m.named_parameters()}

@@ -1,0 +1,131 @@
+"""Pins the oracle's render functions (oracle/render.py) against closed forms,
+identities and brute-force loops that share no code with it."""
+import math
+
+import numpy as np
+import torch
+
+import npref
+from oracle import alpha_grid_sample, composite, eval_sh, raw2alpha, sample_pdf, sample_point
+
+
+def test_sample_point_closed_form():
+    g = torch.Generator().manual_seed(0)
+    B, Ns = 50, 40
+    o = torch.randn((B, 3), generator=g) * 3
+    d = torch.randn((B, 3), generator=g)
+    d = d / d.norm(dim=-1, keepdim=True)
+    aabb = torch.tensor([[-1.0, -1.2, -0.8], [1.1, 0.9, 1.3]])
+    step = torch.tensor(0.05)
+    u = torch.rand(B, generator=g)
+    pts, t, inner = sample_point(o, d, aabb, step, Ns, u)
+    on, dn, an = o.double().numpy(), d.double().numpy(), aabb.double().numpy()
+    ta = (an[1] - on) / dn
+    tb = (an[0] - on) / dn
+    tmin = np.clip(np.minimum(ta, tb).max(-1), 0.05, 1e3)
+    t_ref = tmin[:, None] + 0.05 * (np.arange(Ns)[None] + u.double().numpy()[:, None])
+    assert np.allclose(t.double().numpy(), t_ref, atol=1e-5)
+    p_ref = on[:, None] + dn[:, None] * t_ref[..., None]
+    assert np.allclose(pts.double().numpy(), p_ref, atol=1e-4)
+    inside = ((p_ref >= an[0]) & (p_ref <= an[1])).all(-1)
+    margin = np.minimum(np.abs(p_ref - an[0]), np.abs(p_ref - an[1])).min(-1) > 1e-4
+    assert np.array_equal(inner.numpy()[margin], inside[margin])
+    # eval mode: no jitter, integer multiples of the step
+    _, t0, _ = sample_point(o, d, aabb, step, Ns, None)
+    assert np.allclose(t0.double().numpy(), tmin[:, None] + 0.05 * np.arange(Ns)[None], atol=1e-5)
+
+
+def test_alpha_grid_sample_vs_bruteforce():
+    g = torch.Generator().manual_seed(1)
+    vol = (torch.rand((5, 6, 7), generator=g) < 0.5).float()
+    aabb = torch.tensor([[-1.0, -0.5, -1.5], [1.0, 1.5, 0.5]])
+    x = aabb[0] + (aabb[1] - aabb[0]) * (torch.rand((300, 3), generator=g) * 1.2 - 0.1)
+    got = alpha_grid_sample(vol, aabb, x.double().float())
+    c = ((x.double() - aabb[0].double()) / (aabb[1] - aabb[0]).double() * 2 - 1).numpy()
+    ref = npref.grid_sample(vol[None].numpy(), c, align_corners=True, padding="zeros")[:, 0]
+    assert np.allclose(got.numpy(), ref, atol=1e-5)
+
+
+def test_raw2alpha_telescoping_and_special_cases():
+    g = torch.Generator().manual_seed(2)
+    sigma = torch.rand((20, 33), generator=g, dtype=torch.float64) * 5
+    sigma[:, ::4] = 0
+    dist = torch.rand((20, 33), generator=g, dtype=torch.float64) * 0.3
+    alpha, w, T_last = raw2alpha(sigma, dist)
+    f = 1 - alpha + 1e-10
+    T = torch.cumprod(torch.cat([torch.ones_like(f[:, :1]), f], -1), -1)
+    # sum(w) + T_last = 1 + 1e-10 * sum_i T_i
+    assert torch.allclose(w.sum(-1) + T_last[:, 0], 1 + 1e-10 * T[:, :-1].sum(-1), atol=1e-12)
+    # w_i = alpha_i * prod_{j<i} (1 - alpha_j + 1e-10), by an explicit loop
+    for r in range(3):
+        run = 1.0
+        for i in range(33):
+            a = 1 - math.exp(-float(sigma[r, i]) * float(dist[r, i]))
+            assert abs(float(w[r, i]) - a * run) < 1e-12
+            run *= 1 - a + 1e-10
+    a1, w1, T1 = raw2alpha(torch.tensor([[2.0]], dtype=torch.float64), torch.tensor([[0.5]], dtype=torch.float64))
+    assert torch.allclose(w1, a1) and torch.allclose(T1, 1 - a1 + 1e-10)
+    a0, w0, T0 = raw2alpha(torch.zeros((1, 5), dtype=torch.float64), torch.ones((1, 5), dtype=torch.float64))
+    assert torch.all(w0 == 0) and torch.all(a0 == 0)
+
+
+def test_composite_white_background():
+    sigma = torch.zeros((4, 10), dtype=torch.float64)
+    rgb = torch.rand((4, 10, 3), dtype=torch.float64)
+    z = torch.linspace(0, 1, 10, dtype=torch.float64).expand(4, 10)
+    dists = torch.full((4, 10), 0.1, dtype=torch.float64)
+    rgb_map, depth, w, acc = composite(sigma, rgb, z, dists, 25.0, True)
+    assert torch.allclose(rgb_map, torch.ones_like(rgb_map)) and torch.all(depth == 0)
+    rgb_map, _, _, _ = composite(sigma, rgb, z, dists, 25.0, False)
+    assert torch.all(rgb_map == 0)
+    # one opaque sample returns its colour
+    sigma[:, 3] = 1e6
+    rgb_map, depth, _, acc = composite(sigma, rgb, z, dists, 25.0, True)
+    assert torch.allclose(rgb_map, rgb[:, 3].clamp(0, 1), atol=1e-8)
+    assert torch.allclose(depth, z[:, 3], atol=1e-8)
+
+
+def test_sh_degree2_orthonormal():
+    # Gauss-Legendre in cos(theta) x uniform phi quadrature
+    nt, nphi = 24, 48
+    xs, ws = np.polynomial.legendre.leggauss(nt)
+    phi = (np.arange(nphi) + 0.5) * 2 * np.pi / nphi
+    ct, ph = np.meshgrid(xs, phi, indexing="ij")
+    st = np.sqrt(1 - ct ** 2)
+    dirs = torch.tensor(np.stack([st * np.cos(ph), st * np.sin(ph), ct], -1).reshape(-1, 3))
+    wq = torch.tensor((ws[:, None] * np.full(nphi, 2 * np.pi / nphi)[None]).reshape(-1))
+    Y = torch.stack([eval_sh(2, torch.eye(9, dtype=torch.float64)[k].expand(len(dirs), 1, 9), dirs)[:, 0]
+                     for k in range(9)], -1)
+    G = (Y * wq[:, None]).T @ Y
+    assert torch.allclose(G, torch.eye(9, dtype=torch.float64), atol=1e-6)
+
+
+def test_sample_pdf_uniform_and_bruteforce():
+    bins = torch.linspace(2.0, 6.0, 9, dtype=torch.float64).expand(3, 9).contiguous()
+    w = torch.ones((3, 8), dtype=torch.float64)
+    s = sample_pdf(bins, w, 17, det=True)
+    assert torch.allclose(s, torch.linspace(2.0, 6.0, 17, dtype=torch.float64).expand(3, 17), atol=1e-9)
+    g = torch.Generator().manual_seed(4)
+    bins = torch.sort(torch.rand((5, 11), generator=g, dtype=torch.float64) * 4, -1)[0]
+    w = torch.rand((5, 10), generator=g, dtype=torch.float64)
+    w[:, 3] = 0
+    u = torch.sort(torch.rand((5, 13), generator=g, dtype=torch.float64), -1)[0]
+    got = sample_pdf(bins, w, 13, u=u)
+    for r in range(5):
+        ww = [float(v) + 1e-5 for v in w[r]]
+        tot = sum(ww)
+        cdf = [0.0]
+        for v in ww:
+            cdf.append(cdf[-1] + v / tot)
+        for k in range(13):
+            uu = float(u[r, k])
+            above = 0
+            while above < len(cdf) and cdf[above] <= uu:
+                above += 1
+            below = max(above - 1, 0)
+            above = min(above, len(cdf) - 1)
+            den = cdf[above] - cdf[below]
+            den = 1.0 if den < 1e-5 else den
+            t = (uu - cdf[below]) / den
+            ref = float(bins[r, below]) + t * (float(bins[r, above]) - float(bins[r, below]))
+            assert abs(float(got[r, k]) - ref) < 1e-9
