@@ -124,64 +124,161 @@ def nerf_setup(dev, pool_rays=1 << 20, seed_off=0):
     return scene, rays, rgb
 
 
-def run_nerf(args):
-    """Per-scene NeRF training, plain hand-written FP32 form: uniform sampler
-    with a 128^3 occupancy grid, 3-D grid basis + 32^3x18 coefficient grid,
-    decoder 18-64-28 with the SH colour head, 4,096 rays per batch."""
+def nerf_bytes_per_sample(model, table_bytes):
+    """Algorithmic HBM bytes per evaluated sample of nerf_field_bwd_kernel:
+    position (12 B) + direction (12 B) + the sample's upstream gradient
+    (sigma, rgb: 16 B) + one read of every gathered table value
+    (`table_bytes` each) + read-modify-write of its FP32 gradient (8 B each).
+    G = 8 corners per basis channel + 8 corners per coefficient channel."""
+    C = sum(model.basis_dims)
+    G = 8 * C + 8 * C
+    return 12 + 12 + 16 + table_bytes * G + 8 * G, G
+
+
+def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pretrain, check_rays=256):
+    """Train one NeRF form and time `timed_steps` steps after `pretrain`.
+
+    form 'plain': FP32 tables, uniform sampler with the 128^3 occupancy grid
+    (train_per_scene.py:126-200 step order: sample, field, composite, MSE,
+    backward, Adam).  form 'proposal': BF16 table copies read by the
+    kernels (FP32 masters updated by Adam), the learned proposal placing 48
+    fine samples per ray and trained on the field's rendering weights each
+    step; once the occupancy grid exists, the occupancy sampler renders the
+    first `check_rays` rays of every batch beside the proposal and the
+    proposal is switched off (training falls back to the occupancy sampler)
+    if the accumulated PSNR gap exceeds 0.1 dB."""
     import torch
     from ffields import FactorFields, FusedAdam
     from ffields import workloads as W
     from ffields.params import N_to_reso, cal_n_samples
 
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-    log("nerf: ground-truth ray pool")
-    scene, pool, pool_rgb = nerf_setup(dev)
-    _, test_rays, test_rgb = nerf_setup(dev, pool_rays=1 << 14, seed_off=7)
     cfg = W.nerf_cfg()
-    model = FactorFields(cfg, dev)
+    torch.manual_seed(W.BASE_SEED)
+    prop = form == "proposal"
+    model = FactorFields(cfg, dev, table_dtype="bf16" if prop else "fp32")
     B = int(cfg.training.batch_size)
     reso = N_to_reso(cfg.training.volume_resoInit ** 3, model.aabb.cpu())
     n_samples = min(int(cfg.renderer.max_samples), cal_n_samples(reso, cfg.renderer.step_ratio))
-    opt = FusedAdam(model.get_optparam_groups(cfg.training.lr_small, cfg.training.lr_large), betas=(0.9, 0.99))
+    if prop:
+        model.enable_proposal(reso=64, n_fine=48, table_dtype="bf16")
+    opt = FusedAdam(model.get_optparam_groups(cfg.training.lr_small, cfg.training.lr_large), betas=(0.9, 0.99),
+                    shadow_of=model.shadow_lookup)
     lr_factor = 0.1 ** (1 / cfg.training.n_iters)
     gen = torch.Generator(device=dev).manual_seed(W.BASE_SEED + 2001)
-    counts = torch.zeros(max(args.steps, 1), dtype=torch.int32, device=dev)
+    counts = torch.zeros(max(timed_steps, 1), dtype=torch.int32, device=dev)
+    sq_err = torch.zeros(2, device=dev)       # squared error of: occupancy sampler, proposal
+    state = {"active": prop, "fallback_at": None, "gaps": []}
 
-    def step(k=None, ev=None):
+    def step(k=None):
         idx = torch.randint(0, pool.shape[0], (B,), device=dev, generator=gen)
-        model.render_train_step(pool[idx], pool_rgb[idx], white_bg=True, N_samples=n_samples)
+        rays, rgb = pool[idx], pool_rgb[idx]
+        if state["active"]:
+            model.render_train_step_proposal(rays, rgb, white_bg=True, reduce=False)
+            if model.alphaMask is not None:
+                jr, jt = rays[:check_rays], rgb[:check_rays]
+                sq_err[0] += ((model.render_eval(jr, use_proposal=False, N_samples=n_samples) - jt) ** 2).sum()
+                sq_err[1] += ((model.render_eval(jr, use_proposal=True) - jt) ** 2).sum()
+        else:
+            model.render_train_step(rays, rgb, white_bg=True, N_samples=n_samples, reduce=False)
         if k is not None:
             counts[k] = model._rbuf[1]["ray_off"][-1]
         opt.step(zero_grad=True)
         for gp in opt.param_groups:
             gp["lr"] *= lr_factor
 
-    log(f"nerf: {args.nerf_pretrain} pre-training steps, then occupancy grid")
-    for _ in range(args.nerf_pretrain):
+    def compare_samplers(it):
+        s0, s1 = sq_err.tolist()
+        sq_err.zero_()
+        if s0 > 0 and s1 > 0:
+            gap = 10 * math.log10(s1 / s0)       # PSNR(occupancy sampler) - PSNR(proposal), dB
+            state["gaps"].append(round(gap, 4))
+            if gap > 0.1:
+                state["active"] = False
+                state["fallback_at"] = it
+
+    upd = pretrain // 2
+    for it in range(pretrain):
         step()
-    model.updateAlphaMask(tuple(reso), is_update_alphaMask=True)
-    occ_frac = float(model.alphaMask.occ.float().mean())
-    for _ in range(args.warmup):
-        step()
+        if it == upd:
+            model.updateAlphaMask(tuple(reso), is_update_alphaMask=True)
+        if state["active"] and it > upd and (it - upd) % 100 == 0:
+            compare_samplers(it)
+    if state["active"] and model.alphaMask is not None:
+        compare_samplers(pretrain)
+    occ_frac = float(model.alphaMask.occ.float().mean()) if model.alphaMask is not None else 1.0
     torch.cuda.synchronize()
-    log(f"nerf: timing {args.steps} steps")
+    model.kernel_events = []
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(timed_steps):
         step(k)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    samples = int(counts[:args.steps].sum())
+    kern = model.kernel_events
+    model.kernel_events = None
+    kern_ms = sum(a.elapsed_time(b) for a, b in kern) / max(len(kern), 1)
+    samples = int(counts[:timed_steps].sum())
     with torch.no_grad():
-        out = []
-        for k in range(0, test_rays.shape[0], 4096):
-            out.append(model(test_rays[k:k + 4096], white_bg=True, is_train=False, N_samples=n_samples)[0])
+        out = [model.render_eval(test_rays[k:k + 4096], use_proposal=state["active"], N_samples=n_samples)
+               for k in range(0, test_rays.shape[0], 4096)]
         mse = torch.mean((torch.cat(out) - test_rgb) ** 2).item()
-    return {"rays_per_s": args.steps * B / dt, "field_pts_per_s": samples / dt,
-            "ms_per_step": dt / args.steps * 1e3, "samples_per_ray": samples / (args.steps * B),
-            "n_samples": n_samples, "occupancy_fraction": occ_frac,
-            "train_steps_total": args.nerf_pretrain + args.warmup + args.steps,
-            "test_psnr": -10 * math.log10(mse), "params": model.n_parameters()}
+    bps, G = nerf_bytes_per_sample(model, 2 if prop else 4)
+    samples_per_launch = samples / max(timed_steps, 1)
+    return {"rays_per_s": timed_steps * B / dt, "field_pts_per_s": samples / dt,
+            "ms_per_step": dt / timed_steps * 1e3, "samples_per_ray": samples / (timed_steps * B),
+            "test_psnr": -10 * math.log10(mse), "proposal_active": state["active"],
+            "fallback_at_step": state["fallback_at"], "sampler_gaps_db": state["gaps"][-5:],
+            "train_steps": pretrain + timed_steps, "occupancy_fraction": occ_frac, "n_samples": n_samples,
+            "table_dtype": "bf16" if prop else "fp32", "params": model.n_parameters(),
+            "field_bwd_ms": kern_ms, "field_bwd_bytes_per_sample": bps, "gathered_values_per_sample": G,
+            "field_bwd_gbs": bps * samples_per_launch / (kern_ms * 1e-3) / 1e9 if kern_ms else None}
+
+
+def run_nerf(args):
+    """NeRF-style per-scene training on the synthetic Gaussian scene: the
+    proposal+BF16 form is the measured value, the plain FP32 form (same
+    scene, model, batch and step count) is timed beside it."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    log("nerf: ground-truth ray pool")
+    _, pool, pool_rgb = nerf_setup(dev)
+    _, test_rays, test_rgb = nerf_setup(dev, pool_rays=1 << 14, seed_off=7)
+    forms = ["plain"] if args.nerf_form == "plain" else ["plain", "proposal"]
+    res = {}
+    for f in forms:
+        log(f"nerf {f}: {args.nerf_pretrain} training steps, then {args.steps} timed")
+        res[f] = nerf_train(f, dev, pool, pool_rgb, test_rays, test_rgb, args.steps, args.nerf_pretrain)
+        log(f"nerf {f}: {res[f]['rays_per_s']:.0f} rays/s, PSNR {res[f]['test_psnr']:.2f} dB")
+    main = res[forms[-1]]
+    out = {
+        "metric": METRIC,
+        "value": main["rays_per_s"],
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.nerf_pretrain,
+        "ms_per_step": main["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (48 seeded anisotropic Gaussians, sinusoid colour; ground truth by 1,024-sample "
+                "quadrature on random rays from 100 radius-4 cameras)",
+        "config": {"workload": ("nerf_proposal_bf16" if forms[-1] == "proposal" else "nerf_plain_fp32")
+                   + ": 4,096 rays/batch, 3-D grid basis [4,4,4,2,2,2]@[32..128], 32^3x18 coefficient grid, "
+                     "18-64-28 decoder + degree-2 SH head",
+                   "field_pts_per_s": main["field_pts_per_s"], "forms": res},
+    }
+    if "proposal" in res:
+        out["config"]["speedup_vs_plain_fp32"] = res["proposal"]["rays_per_s"] / res["plain"]["rays_per_s"]
+        out["config"]["psnr_gap_db"] = res["plain"]["test_psnr"] - res["proposal"]["test_psnr"]
+    bw = main["field_bwd_gbs"]
+    out["roofline"] = {"bound": "hbm", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": bw / HBM_PEAK_GBS if bw else None, "traffic": None,
+                       "kernel": "nerf_field_bwd_kernel (+ its decoder-gradient reduction)",
+                       "kernel_ms": main["field_bwd_ms"], "bytes_per_sample": main["field_bwd_bytes_per_sample"]}
+    return out
 
 
 def run_image(args):
@@ -271,15 +368,71 @@ def cpu_baseline_sdf(args):
                       f"oracle (torch CPU, grid_sample per level + autograd + Adam)"}
 
 
+def cpu_baseline_nerf(args):
+    """The oracle NeRF path (same ATen ops as the reference: sample_point,
+    grid_sample per level, linear_mat, raw2alpha, SH head, autograd,
+    torch.optim.Adam) on the host cores, plain form without an occupancy
+    grid, on a bounded batch of rays of the same scene."""
+    import torch
+    from ffields import workloads as W
+    from ffields.params import N_to_reso, cal_n_samples
+    from oracle import OracleField
+    from oracle.render import OracleRenderer
+
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    cfg = W.nerf_cfg()
+    of = OracleField(cfg).requires_grad_(True)
+    aabb = of.aabb
+    reso = N_to_reso(cfg.training.volume_resoInit ** 3, aabb)
+    units = (aabb[1] - aabb[0]) / (torch.tensor(reso, dtype=torch.float32) - 1)
+    step = float(units.mean() * cfg.renderer.step_ratio)
+    n_samples = min(int(cfg.renderer.max_samples), cal_n_samples(reso, cfg.renderer.step_ratio))
+    rd = OracleRenderer(of, step_size=step, n_samples=n_samples)
+    opt = torch.optim.Adam([{"params": [w for w, _ in of.mlp] + [b for _, b in of.mlp if b is not None],
+                             "lr": cfg.training.lr_small},
+                            {"params": of.coeffs + of.basises, "lr": cfg.training.lr_large}],
+                           betas=(0.9, 0.99))
+    scene = W.NerfScene(device="cpu")
+    g = torch.Generator().manual_seed(W.BASE_SEED + 2000)
+    n = args.cpu_rays
+    rays = scene.random_rays(n, g)
+    rgb = scene.render_gt(rays)
+
+    def one():
+        out = rd.forward(rays, white_bg=True, n_samples=n_samples)[0]
+        loss = torch.mean((out - rgb) ** 2)
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+
+    one()
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 3 or (time.perf_counter() - t_start < 10 and len(times) < 20):
+        t0 = time.perf_counter()
+        one()
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > 30:
+            break
+    times.sort()
+    med = times[len(times) // 2]
+    return {"value": n / med, "unit": "rays/s", "cores": cores, "kind": "port",
+            "sample": f"{len(times)} timed steps (median) of {n} rays of the same scene and model, plain form "
+                      f"without occupancy grid ({n_samples} samples/ray), oracle (torch CPU, autograd + Adam)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="sdf", choices=["sdf", "nerf", "image"])
+    ap.add_argument("--workload", default="nerf", choices=["sdf", "nerf", "image"])
+    ap.add_argument("--nerf-form", default="both", choices=["both", "plain"])
     ap.add_argument("--image-iters", type=int, default=2000)
     ap.add_argument("--nerf-pretrain", type=int, default=300)
     ap.add_argument("--cpu-batch", type=int, default=1 << 15)
+    ap.add_argument("--cpu-rays", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -288,7 +441,11 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     if args.workload == "nerf":
-        print(json.dumps(run_nerf(args)), flush=True)
+        result = run_nerf(args)
+        if rank == 0 and not args.no_cpu_baseline:
+            log("cpu baseline")
+            result["cpu_baseline"] = cpu_baseline_nerf(args)
+        print(json.dumps(result), flush=True)
         return
     if args.workload == "image":
         print(json.dumps(run_image(args)), flush=True)

@@ -21,8 +21,8 @@
 
 namespace ff {
 
-// basis_dims [4,4,4,2,2,2]: the layout of every BASELINE workload but the
-// dictionary image set.
+// basis_dims [4,4,4,2,2,2]: the basis layout of the image, SDF and NeRF
+// workloads (18 channels over 6 levels).
 struct Spec18 {
   static constexpr int L = 6;
   static constexpr int C = 18;

@@ -181,7 +181,7 @@ class RenderMixin:
     def enable_proposal(self, reso=64, n_fine=48, table_dtype="bf16", density_shift=-5.0, seed=0):
         """Attach a ProposalNet (64^3 x 8 grid + MLP 8-32-1) that places n_fine
         samples per ray; the occupancy sampler stays available as fallback
-        and judge (use_proposal toggles which one trains)."""
+        and reference sampler (use_proposal toggles which one trains)."""
         from .proposal import ProposalNet
         self.proposal = ProposalNet(self.aabb.cpu(), reso=reso, n_fine=n_fine, density_shift=density_shift,
                                     distance_scale=float(self.cfg.renderer.distance_scale),
@@ -225,9 +225,16 @@ class RenderMixin:
         g = self._train_grads()
         ws_bytes = L.lib().ff_render_field_bwd_workspace_bytes(ctypes.byref(d), ctypes.byref(m), s.cap)
         ws = self._workspace(ws_bytes)
+        ev = getattr(self, "kernel_events", None)
+        if ev is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
         L.check(L.lib().ff_render_field_bwd(ctypes.byref(d), ctypes.byref(m), ctypes.byref(h), L.ptr(rays),
                                             B, ctypes.byref(s), L.ptr(buf["d_sample"]), ctypes.byref(g),
                                             L.ptr(ws), ws_bytes, st), "ff_render_field_bwd")
+        if ev is not None:
+            e1.record()
+            ev.append((e0, e1))
         return rgb_map
 
     def render_train_step_proposal(self, rays, target, white_bg=True, loss_scale=1.0, prop_loss_scale=1.0,

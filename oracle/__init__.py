@@ -6,35 +6,44 @@ algorithm (autonomousvision/factor-fields, ``models/FactorFields.py``,
 same ATen operations and the same operation order as the reference, so that it
 reproduces the reference's FP32 semantics on the same inputs.  Only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
-import it, and only as the checker or the timed CPU baseline -- never as part of
+import it, and only to check results or as the timed CPU baseline -- never as part of
 the product path, which must fail loudly when its HIP library is missing.
 
 How each function is pinned (checked against something other than itself):
 
 * ``grid_mapping``      -- closed forms, periodicity / range invariants and the
-                           identity special case (tests/test_oracle_mapping.py).
+                           identity special case (tests/test_oracle_field.py).
 * ``dct_dict``          -- closed-form DCT rows, selected-row tables and unit
-                           row norms (tests/test_oracle_init.py).
+                           row norms (tests/test_oracle_field.py).
 * ``setup_params``      -- the parameter counts the reference notebooks printed
                            (1,429,376 for configs/image.yaml; 5,342,274 for
-                           configs/sdf.yaml) and hand arithmetic.
+                           configs/sdf.yaml) and hand arithmetic
+                           (tests/test_oracle_field.py).
 * ``grid_sample`` uses  -- an independent numpy brute-force bi/trilinear and
-                           nearest interpolator (tests/test_oracle_gather.py),
-                           partition of unity, FP64 finite differences for the
-                           gradients.
+                           nearest interpolator (tests/npref.py), partition of
+                           unity, FP64 central differences for the gradients
+                           (tests/test_oracle_field.py).
 * VM coefficient        -- new semantics (the reference's ``vm`` coefficient
                            cannot be combined with a grid basis); pinned by the
                            same numpy brute force on planes and lines.
-* ``MLPMixer``          -- FP64 matrix products in numpy.
+* ``MLPMixer``          -- FP64 matrix products in numpy (tests/npref.py).
 * sampler / occupancy   -- closed forms with injected jitter; exact integer
-                           properties (monotone t, in-box masks).
+                           properties (monotone t, in-box masks); brute-force
+                           trilinear occupancy (tests/test_oracle_render.py).
 * ``raw2alpha``/composite -- telescoping identity sum(w) + T_last = prod
                            (with the +1e-10 term), single-sample and sigma = 0
-                           special cases, FP64 finite differences.
+                           special cases, FP64 central differences
+                           (tests/test_oracle_render.py).
 * ``eval_sh``           -- orthonormality of the degree-2 real SH basis by
                            quadrature on the sphere.
 * ``sample_pdf``        -- uniform-weights special case (stratified lerp) and a
                            pure-Python inverse-CDF loop on tiny inputs.
+* proposal              -- explicit-overlap loops for the bound loss, its zero
+                           set and FP64 central differences
+                           (tests/test_oracle_proposal.py).
+
+tests/golden/ holds fixtures written from these functions
+(tests/golden/make_golden.py); tests/test_golden.py recomputes them.
 """
 
 from .field import (AttrDict, N_to_reso, N_to_vm_reso, OracleField, cal_n_samples,

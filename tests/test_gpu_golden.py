@@ -1,0 +1,113 @@
+"""HIP path against the committed golden fixtures (tests/golden/*.npz).
+
+Bars: field features/outputs within 1e-5 relative of the FP64 fixture
+(absolute floor 1e-6 of the max); render sample counts and candidate indices
+bit-exact, sigma/rgb_map/depth within 1e-5 relative; proposal fine samples
+within 1e-5 of the ray's span except at most 1% of them (an FP32 rounding
+difference at a CDF step moves a sample to the neighbouring bin)."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import assert_close
+from ffields import FactorFields
+from ffields.config import AttrDict
+from ffields.proposal import ProposalNet
+from ffields.render import AlphaGridMask, _samples_struct, alloc_samples
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIELD_CASES = ["field_2d_grid", "field_2d_triangle_nearest", "field_3d_grid", "field_3d_vm"]
+
+
+def load(name):
+    with np.load(os.path.join(HERE, "golden", name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def model_from(fx):
+    m = FactorFields(AttrDict.wrap(json.loads(str(fx["cfg"]))), "cuda")
+    with torch.no_grad():
+        for k, p in enumerate(m.coeffs):
+            p.copy_(torch.from_numpy(fx[f"coeff{k}"]).cuda())
+        for k, p in enumerate(m.basises):
+            p.copy_(torch.from_numpy(fx[f"basis{k}"]).cuda())
+        lin = m.linear_mat.backbone
+        for k in range(len(lin)):
+            lin[k].weight.copy_(torch.from_numpy(fx[f"w{k}"]).cuda())
+            if lin[k].bias is not None:
+                lin[k].bias.copy_(torch.from_numpy(fx[f"b{k}"]).cuda())
+    return m
+
+
+@pytest.mark.parametrize("name", FIELD_CASES)
+def test_field_matches_fixture(name):
+    print(f"[case] golden {name}", file=sys.stderr, flush=True)
+    fx = load(name)
+    m = model_from(fx)
+    x = torch.from_numpy(fx["x"]).cuda()
+    with torch.no_grad():
+        feats, coeff = m.get_coding(x)
+        out = m.linear_mat(feats)
+    assert_close(feats, torch.from_numpy(fx["feats"]), 1e-5, "feats")
+    assert_close(coeff, torch.from_numpy(fx["coeff"]), 1e-5, "coeff")
+    assert_close(out, torch.from_numpy(fx["out"]), 1e-5, "out")
+
+
+def test_render_matches_fixture():
+    print("[case] golden render", file=sys.stderr, flush=True)
+    fx = load("render")
+    m = model_from(fx)
+    m.alphaMask = AlphaGridMask(torch.device("cuda"), torch.from_numpy(fx["occ_aabb"]), torch.from_numpy(fx["occ"]))
+    assert m._step == float(fx["step"]) and m.nSamples == int(fx["n_samples"])
+    rays = torch.from_numpy(fx["rays"]).cuda()
+    jit = torch.from_numpy(fx["jitter"]).cuda()
+    B, Ns = rays.shape[0], m.nSamples
+    buf = m.render_buffers(B, Ns)
+    m.render_samples(rays, Ns, jit, buf)
+    with torch.no_grad():
+        rgb, depth, _ = m(rays, white_bg=True, is_train=False, jitter=jit)
+    torch.cuda.synchronize()
+    off = buf["ray_off"].cpu()
+    M = int(off[-1])
+    assert np.array_equal((off[1:] - off[:-1]).numpy(), fx["counts"])
+    assert np.array_equal(buf["idx"][:M].cpu().numpy(), fx["idx"])
+    assert_close(buf["sigma"][:M], torch.from_numpy(fx["sigma"]), 1e-5, "sigma")
+    assert_close(rgb, torch.from_numpy(fx["rgb_map"]), 1e-5, "rgb_map")
+    assert_close(depth, torch.from_numpy(fx["depth"]), 1e-5, "depth")
+
+
+@pytest.mark.parametrize("train", [True, False])
+def test_proposal_matches_fixture(train):
+    print(f"[case] golden proposal train={train}", file=sys.stderr, flush=True)
+    fx = load("proposal")
+    p = ProposalNet(fx["aabb"].tolist(), reso=9, n_fine=48, density_shift=-1.0, table_dtype="fp32", device="cuda")
+    with torch.no_grad():
+        p.grid.copy_(torch.from_numpy(fx["grid"]).cuda())
+        p.mlp[0].weight.copy_(torch.from_numpy(fx["w0"]).cuda())
+        p.mlp[0].bias.copy_(torch.from_numpy(fx["b0"]).cuda())
+        p.mlp[1].weight.copy_(torch.from_numpy(fx["w1"]).cuda())
+    rays = torch.from_numpy(fx["rays"]).cuda()
+    B = rays.shape[0]
+    buf = alloc_samples(B, 48, "cuda")
+    s = _samples_struct(buf)
+    if train:
+        p.sample(rays, s, torch.from_numpy(fx["prop_u"]).cuda(), torch.from_numpy(fx["fine_u"]).cuda())
+        t_ref = fx["t_fine"]
+    else:
+        p.sample(rays, s)
+        t_ref = fx["t_fine_det"]
+    torch.cuda.synchronize()
+    hit = fx["hit"]
+    off = buf["ray_off"].cpu().numpy()
+    assert np.array_equal(off[1:] - off[:-1], hit.astype(np.int32) * 48)
+    M = int(off[-1])
+    t = buf["z"][:M].cpu().numpy().reshape(-1, 48)
+    span = (fx["bins"][hit, -1] - fx["bins"][hit, 0])[:, None]
+    bad = np.abs(t - t_ref[hit]) > 1e-5 * span + 1e-6
+    assert int(bad.sum()) <= max(1, int(0.01 * bad.size)), int(bad.sum())

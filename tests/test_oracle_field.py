@@ -135,7 +135,7 @@ def test_param_count_sdf_notebook():
 
 
 def test_nerf_literal_via_reference_keys():
-    """configs[2]'s literal basis resos are reachable via reference keys:
+    """The NeRF workload's literal basis resos are reachable via reference keys:
     total_params 10,747,166, coeff_reso 32, scene_reso 1024."""
     cfg = make_cfg("reconstruction", dict(total_params=10747166, coeff_reso=32,
                                           freq_bands=[2.0, 3.2, 4.4, 5.6, 6.8, 8.0]),

@@ -85,6 +85,50 @@ def test_composite_white_background():
     assert torch.allclose(depth, z[:, 3], atol=1e-8)
 
 
+def test_composite_gradient_fp64_central_differences():
+    """d rgb_map / d sigma and d rgb_map / d rgb of the composite (the
+    quantities the hand-written composite backward produces) against FP64
+    central differences of an explicit-loop composite."""
+    g = torch.Generator().manual_seed(9)
+    sigma = torch.rand((3, 12), generator=g, dtype=torch.float64) * 3
+    rgb = torch.rand((3, 12, 3), generator=g, dtype=torch.float64) * 0.8 + 0.1
+    z = torch.cumsum(torch.rand((3, 12), generator=g, dtype=torch.float64) * 0.05 + 0.01, -1)
+    dists = torch.cat([z[:, 1:] - z[:, :-1], torch.zeros_like(z[:, :1])], -1)
+    wt = torch.rand((3, 3), generator=g, dtype=torch.float64)
+
+    def loop_loss(sg, cl):
+        tot = 0.0
+        for r in range(3):
+            run, acc, out = 1.0, 0.0, [0.0, 0.0, 0.0]
+            for i in range(12):
+                a = 1 - math.exp(-float(sg[r, i]) * float(dists[r, i]) * 25.0)
+                w = a * run
+                acc += w
+                for c in range(3):
+                    out[c] += w * float(cl[r, i, c])
+                run *= 1 - a + 1e-10
+            for c in range(3):
+                tot += float(wt[r, c]) * min(max(out[c] + (1 - acc), 0.0), 1.0)
+        return tot
+
+    s_ = sigma.clone().requires_grad_(True)
+    c_ = rgb.clone().requires_grad_(True)
+    rgb_map, _, _, _ = composite(s_, c_, z, dists, 25.0, True)
+    gs, gc = torch.autograd.grad((rgb_map * wt).sum(), [s_, c_])
+    eps = 1e-6
+    for (r, i) in [(0, 0), (1, 5), (2, 11), (0, 7)]:
+        sp, sm = sigma.clone(), sigma.clone()
+        sp[r, i] += eps
+        sm[r, i] -= eps
+        fd = (loop_loss(sp, rgb) - loop_loss(sm, rgb)) / (2 * eps)
+        assert abs(fd - float(gs[r, i])) <= 1e-6 * max(1.0, abs(fd)), (r, i, fd, float(gs[r, i]))
+        cp, cm = rgb.clone(), rgb.clone()
+        cp[r, i, 1] += eps
+        cm[r, i, 1] -= eps
+        fd = (loop_loss(sigma, cp) - loop_loss(sigma, cm)) / (2 * eps)
+        assert abs(fd - float(gc[r, i, 1])) <= 1e-6 * max(1.0, abs(fd)), (r, i)
+
+
 def test_sh_degree2_orthonormal():
     # Gauss-Legendre in cos(theta) x uniform phi quadrature
     nt, nphi = 24, 48
