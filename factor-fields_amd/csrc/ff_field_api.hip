@@ -7,7 +7,6 @@
 
 namespace ff {
 
-constexpr int kReduceSplit = 32;
 
 // Sum workspace rows [rows, P] into the decoder gradients in a fixed order:
 // stage 1 sums row slices r = y, y + 32, ... into part2[y][k]; stage 2 sums the

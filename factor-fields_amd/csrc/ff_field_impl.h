@@ -250,38 +250,69 @@ __device__ __forceinline__ void coding_bwd(const ff_field_desc& fd, const ff_gra
 
 // ------------------------------------------------------------ coalesced scatter
 // Atomic throughput is bounded by cache-line requests, not by the lanes each
-// request carries, so the backward does not let every lane scatter its own
-// point: each lane stages its corners (texel, weight) and gradient vector in
-// a per-wave LDS area, then groups of d consecutive lanes add one texel's d
-// contiguous channels, the corners of a point one after the other (x
-// neighbours share a line).  Staging layout (floats): G [64*d] | W [64*K] |
-// T [64*K] (int bits).  Every lane of the wave must call wave_scatter.
-constexpr int kStageWords = 64 * 18 + 64 * 8 * 2;
+// request carries, and atomics to one address serialise, so the backward does
+// not let every lane scatter its own point.  Each lane stages its corners
+// (texel, weight) and gradient vector in a per-wave LDS area; for each corner
+// slot k, consecutive points whose corner k is the same texel (samples along
+// a ray, neighbouring query points) form a run, and only the first point of
+// each run enters a head list (wave prefix sum of per-point head counts).  Groups of d
+// consecutive lanes then take one head each, sum w * grad over its run in
+// registers and add the texel's d contiguous channels with one atomic per
+// channel.  Staging layout (words): G [64*d] | W [64*K] | T [64*K] |
+// heads [64*K].  Every lane of the wave must call wave_scatter, with the same
+// K and d.
+constexpr int kStageWords = 64 * 18 + 64 * 8 * 3;
 
 template <int K, int d>
 __device__ __forceinline__ void wave_scatter(float* stage, float* dst, const int* tex, const float* w,
                                              const float* g, bool active) {
-  static_assert(d <= 64 && 64 * d + 2 * 64 * K <= kStageWords, "staging area too small");
+  static_assert(d <= 64 && K <= 16 && 64 * d + 3 * 64 * K <= kStageWords, "staging area too small");
   const int lane = threadIdx.x & 63;
   float* G = stage;
   float* W = stage + 64 * d;
   int* T = reinterpret_cast<int*>(stage + 64 * d + 64 * K);
+  int* H = T + 64 * K;
 #pragma unroll
-  for (int c = 0; c < d; ++c) G[lane * d + c] = g[c];
+  for (int c = 0; c < d; ++c) G[lane * d + c] = active ? g[c] : 0.f;
+  int hm = 0;  // corner slots where this point starts a run
 #pragma unroll
   for (int k = 0; k < K; ++k) {
+    const int t = active ? tex[k] : -1;  // inactive lanes: a texel no run shares
     W[lane * K + k] = active ? w[k] : 0.f;
-    T[lane * K + k] = tex[k];
+    T[lane * K + k] = t;
+    const int tp = __shfl_up(t, 1);
+    hm |= int(t >= 0 && (lane == 0 || t != tp)) << k;
   }
+  // heads in point-major order (the corners of a point stay adjacent, so x
+  // neighbours share cache lines within one atomic instruction)
+  const int cnt = __popc(hm);
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  const int nh = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+  int pos = incl - cnt;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((hm >> k) & 1) H[pos++] = (lane << 4) | k;
   __syncthreads();
-  constexpr int PPI = 64 / d;  // (point, corner) pairs per instruction
+  constexpr int PPI = 64 / d;  // heads per pass
   const int sub = lane / d, c = lane - sub * d;
-#pragma unroll 2
-  for (int base = 0; base < 64 * K; base += PPI) {
-    const int q = base + sub;  // q = point * K + corner
-    if (sub < PPI && q < 64 * K) {
-      const float ww = W[q];
-      if (ww != 0.f) unsafeAtomicAdd(dst + (int64_t)T[q] * d + c, ww * G[(q / K) * d + c]);
+  for (int base = 0; base < nh; base += PPI) {
+    const int h = base + sub;
+    if (sub < PPI && h < nh) {
+      const int e = H[h];
+      const int k = e & 15;
+      int s = e >> 4;
+      const int t = T[s * K + k];
+      float acc = 0.f;
+      do {
+        acc = fmaf(W[s * K + k], G[s * d + c], acc);
+        ++s;
+      } while (s < 64 && T[s * K + k] == t);
+      if (acc != 0.f) unsafeAtomicAdd(dst + (int64_t)t * d + c, acc);
     }
   }
   __syncthreads();

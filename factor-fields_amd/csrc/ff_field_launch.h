@@ -35,6 +35,10 @@ int launch_mlp_bwd(const ff_mlp_desc& m, const float* f, int64_t n, const float*
 
 // Deterministic reduction of per-wave decoder weight-gradient partials into
 // the gradient buffers (ff_field_api.hip).
+// Deterministic two-level sum of per-wave partial rows [rows][P] (P = decoder
+// parameters of an NL-layer IN -> H -> O MLP) into g.w0 / g.b0 / g.w1; the
+// workspace holds the rows followed by kReduceSplit rows of scratch.
+constexpr int kReduceSplit = 32;
 int launch_reduce_partials(const float* ws, int rows, int NL, int IN, int H, int O, const ff_grads& g,
                            hipStream_t s);
 

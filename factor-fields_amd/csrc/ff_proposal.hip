@@ -14,6 +14,7 @@
 // loss against the field's fine weights, and back-propagates through the
 // proposal compositing (reverse affine scan), the MLP and the grid.
 #include "ff_common.h"
+#include "ff_field_impl.h"
 
 namespace ff {
 
@@ -208,24 +209,37 @@ __global__ __launch_bounds__(256) void proposal_sample_kernel(ff_proposal_desc p
   }
 }
 
-constexpr int kPropParams = kPH * kPC + kPH + kPH;  // dW0, db0, dW1
+constexpr int kPropParams = kPH * kPC + kPH + kPH;  // dW0, db0, dW1 (partial row layout)
+// Per-wave LDS: MLP transpose H [64][33] | F [64][9] | dout [64], reused as
+// the scatter staging area afterwards.
+constexpr int kPropHS = 33, kPropFS = 9;
+constexpr int kPropLds = 64 * kPropHS + 64 * kPropFS + 64;
+constexpr int kPropLdsWords = kPropLds > kStageWords ? kPropLds : kStageWords;
 
+// One 64-thread block = one wavefront; each wave walks rays b = wave,
+// wave + nwaves, ...  MLP weight gradients: lane L owns hidden unit i = L/2
+// and half h = L%2 of the input channels (dW0[i][4h..4h+3]) plus db0[i]
+// (h = 0) or dW1[i] (h = 1), accumulated over the wave's samples from an LDS
+// transpose; each wave writes one partial row, reduced afterwards in a fixed
+// order.  Grid gradients go through the run-merging coalesced scatter.
 template <bool BF>
-__global__ __launch_bounds__(256) void proposal_bwd_kernel(ff_proposal_desc p, const float* __restrict__ rays,
-                                                           const float* __restrict__ prop_u, int64_t B,
-                                                           ff_samples fine, const float* __restrict__ weights,
-                                                           float loss_scale, ff_proposal_grads g,
-                                                           float* __restrict__ loss_sum) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+__global__ __launch_bounds__(64) void proposal_bwd_kernel(ff_proposal_desc p, const float* __restrict__ rays,
+                                                          const float* __restrict__ prop_u, int64_t B,
+                                                          ff_samples fine, const float* __restrict__ weights,
+                                                          float loss_scale, ff_proposal_grads g,
+                                                          float* __restrict__ loss_sum, float* __restrict__ ws) {
+  __shared__ float lds[kPropLdsWords];
+  float* Hs = lds;
+  float* Fs = lds + 64 * kPropHS;
+  float* Ds = Fs + 64 * kPropFS;
+  const int lane = threadIdx.x;
+  const int64_t wave = blockIdx.x;
+  const int64_t nwaves = gridDim.x;
   float* grid_g = g.grid;
   if (g.replicas > 1) grid_g += (int64_t)(xcc_id() % g.replicas) * g.replica_stride;
-  // lane-distributed decoder-gradient accumulators: value i lives in lane i % 64, slot i / 64
-  constexpr int kSlots = (kPropParams + 63) / 64;
-  float acc[kSlots];
-#pragma unroll
-  for (int k = 0; k < kSlots; ++k) acc[k] = 0.f;
+  const int hi = lane >> 1, half = lane & 1;
+  const float w1i = p.w1[hi];
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f;
   float lsum = 0.f;
   const float scale = loss_scale / (float)B;
   for (int64_t b = wave; b < B; b += nwaves) {
@@ -274,7 +288,7 @@ __global__ __launch_bounds__(256) void proposal_bwd_kernel(ff_proposal_desc p, c
     const float dalpha = st.T * (dw - R);
     const float dsig = dalpha * (st.delta * st.ex);
     const float dout = dsig * prop_softplus_grad(st.pre);
-    // MLP backward
+    // MLP backward: d feat per lane (sample); weight gradients via the LDS transpose
     const pcfp w0 = (pcfp)p.w0;
     const pcfp w1 = (pcfp)p.w1;
     float df[kPC];
@@ -285,47 +299,44 @@ __global__ __launch_bounds__(256) void proposal_bwd_kernel(ff_proposal_desc p, c
       const float dh = st.h[i] > 0.f ? w1[i] * dout : 0.f;
 #pragma unroll
       for (int c = 0; c < kPC; ++c) df[c] = fmaf(w0[i * kPC + c], dh, df[c]);
-      // dW0[i][c] += dh f[c]; db0[i] += dh; dW1[i] += dout h[i]
-#pragma unroll
-      for (int c = 0; c < kPC; ++c) {
-        const float s = wave_sum(dh * st.feat[c]);
-        constexpr int dummy = 0;
-        (void)dummy;
-        const int idx = i * kPC + c;
-        if (lane == (idx & 63)) acc[idx >> 6] += s;
-      }
-      {
-        const float s = wave_sum(dh);
-        const int idx = kPH * kPC + i;
-        if (lane == (idx & 63)) acc[idx >> 6] += s;
-      }
-      {
-        const float s = wave_sum(dout * st.h[i]);
-        const int idx = kPH * kPC + kPH + i;
-        if (lane == (idx & 63)) acc[idx >> 6] += s;
-      }
+      Hs[lane * kPropHS + i] = st.h[i];
     }
-    // grid scatter
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (st.cr.w[k] != 0.f) {
-        float* q = grid_g + (int64_t)st.cr.tex[k] * kPC;
-#pragma unroll
-        for (int c = 0; c < kPC; ++c) unsafeAtomicAdd(q + c, st.cr.w[k] * df[c]);
-      }
+    for (int c = 0; c < kPC; ++c) Fs[lane * kPropFS + c] = st.feat[c];
+    Ds[lane] = dout;
+    __syncthreads();
+#pragma unroll 4
+    for (int j = 0; j < 64; ++j) {
+      const float hv = Hs[j * kPropHS + hi];
+      const float d = Ds[j];
+      const float dh = hv > 0.f ? w1i * d : 0.f;
+      const float* f = Fs + j * kPropFS + half * 4;
+      a0 = fmaf(dh, f[0], a0);
+      a1 = fmaf(dh, f[1], a1);
+      a2 = fmaf(dh, f[2], a2);
+      a3 = fmaf(dh, f[3], a3);
+      a4 += half ? d * hv : dh;
     }
+    __syncthreads();
+    // grid scatter (all 64 lanes hold real proposal samples of a hit ray)
+    wave_scatter<8, kPC>(lds, grid_g, st.cr.tex, st.cr.w, df, true);
   }
-#pragma unroll
-  for (int k = 0; k < kSlots; ++k) {
-    const int idx = k * 64 + lane;
-    if (idx < kPropParams && acc[k] != 0.f) {
-      float* dst = idx < kPH * kPC ? g.w0 + idx : (idx < kPH * kPC + kPH ? g.b0 + (idx - kPH * kPC)
-                                                                         : g.w1 + (idx - kPH * kPC - kPH));
-      unsafeAtomicAdd(dst, acc[k]);
-    }
-  }
+  float* row = ws + wave * kPropParams;
+  const int base = hi * kPC + half * 4;
+  row[base + 0] = a0;
+  row[base + 1] = a1;
+  row[base + 2] = a2;
+  row[base + 3] = a3;
+  row[half ? kPH * kPC + kPH + hi : kPH * kPC + hi] = a4;
   lsum = wave_sum(lsum);
   if (lane == 0 && loss_sum) unsafeAtomicAdd(loss_sum, lsum * loss_scale / (float)B);
+}
+
+int64_t prop_bwd_waves(int64_t n_rays) {
+  int cu = 256, dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int64_t cap = (int64_t)cu * 16;
+  return n_rays < cap ? n_rays : cap;
 }
 
 int prop_check(const ff_proposal_desc* p) {
@@ -370,30 +381,39 @@ int ff_proposal_sample(const ff_proposal_desc* p, const float* rays, const float
   return p->table_dtype == FF_BF16 ? go(std::true_type{}) : go(std::false_type{});
 }
 
+size_t ff_proposal_bwd_workspace_bytes(int64_t n_rays) {
+  if (n_rays <= 0) return 0;
+  return (size_t)(prop_bwd_waves(n_rays) + kReduceSplit) * kPropParams * sizeof(float);
+}
+
 int ff_proposal_bwd(const ff_proposal_desc* p, const float* rays, const float* prop_u, int64_t n_rays,
                     const ff_samples* fine, const float* weights, float loss_scale,
-                    const ff_proposal_grads* grads, float* loss_sum, void* stream) {
+                    const ff_proposal_grads* grads, float* loss_sum, void* ws, size_t ws_bytes, void* stream) {
   int st = prop_check(p);
   if (st) return st;
   if (!fine || !fine->ray_off || !fine->z || !weights || !grads || !grads->grid || !grads->w0 || !grads->b0 ||
       !grads->w1 || n_rays < 0 || (n_rays > 0 && !rays))
     return FF_ERR_ARG;
   if (n_rays == 0) return FF_OK;
+  if (!ws || ws_bytes < ff_proposal_bwd_workspace_bytes(n_rays)) return FF_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
-  int cu = 256, dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
-  const int64_t waves = n_rays < (int64_t)cu * 8 ? n_rays : (int64_t)cu * 8;
-  const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
+  const int64_t waves = prop_bwd_waves(n_rays);
   const ff_proposal_desc pd = *p;
   const ff_proposal_grads gd = *grads;
   const ff_samples fs = *fine;
+  float* wsf = (float*)ws;
   if (p->table_dtype == FF_BF16)
-    hipLaunchKernelGGL((proposal_bwd_kernel<true>), dim3(blocks), dim3(256), 0, s, pd, rays, prop_u, n_rays, fs,
-                       weights, loss_scale, gd, loss_sum);
+    hipLaunchKernelGGL((proposal_bwd_kernel<true>), dim3((unsigned)waves), dim3(64), 0, s, pd, rays, prop_u,
+                       n_rays, fs, weights, loss_scale, gd, loss_sum, wsf);
   else
-    hipLaunchKernelGGL((proposal_bwd_kernel<false>), dim3(blocks), dim3(256), 0, s, pd, rays, prop_u, n_rays, fs,
-                       weights, loss_scale, gd, loss_sum);
-  return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
+    hipLaunchKernelGGL((proposal_bwd_kernel<false>), dim3((unsigned)waves), dim3(64), 0, s, pd, rays, prop_u,
+                       n_rays, fs, weights, loss_scale, gd, loss_sum, wsf);
+  if (hipGetLastError() != hipSuccess) return FF_ERR_LAUNCH;
+  ff_grads mg{};
+  mg.w0 = grads->w0;
+  mg.b0 = grads->b0;
+  mg.w1 = grads->w1;
+  return launch_reduce_partials(wsf, (int)waves, 2, kPC, kPH, 1, mg, s);
 }
 
 }  // extern "C"

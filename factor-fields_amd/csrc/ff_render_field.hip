@@ -92,6 +92,7 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff
                                                             float* __restrict__ ws) {
   constexpr int C = Spec18::C;
   using Acc = MlpAcc<C, 64, 28, 2>;
+  static_assert(kWave * 65 >= kStageWords, "LDS tile must also hold the scatter staging area");
   __shared__ float lds[kWave * 65];
   Acc acc;
   acc.zero();
@@ -268,7 +269,7 @@ int ff_render_alpha(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_
 size_t ff_render_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp,
                                            int64_t cap) {
   if (!desc || !mlp || cap <= 0 || !nerf_supported(*desc, *mlp)) return 0;
-  return ((size_t)rf_blocks(cap, 64, 16) + 32) * kNerfPartial * 4;
+  return ((size_t)rf_blocks(cap, 64, 16) + kReduceSplit) * kNerfPartial * 4;
 }
 
 int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
@@ -286,7 +287,7 @@ int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
   if (!grads->w0 || !grads->b0 || !grads->w1) return FF_ERR_ARG;
   if (s->cap <= 0 || n_rays == 0) return FF_OK;
   const unsigned blocks = rf_blocks(s->cap, 64, 16);
-  if (!ws || ws_bytes < ((size_t)blocks + 32) * kNerfPartial * 4) return FF_ERR_WORKSPACE;
+  if (!ws || ws_bytes < ((size_t)blocks + kReduceSplit) * kNerfPartial * 4) return FF_ERR_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   const ff_field_desc d = *desc;
   const ff_mlp_desc m = *mlp;

@@ -124,8 +124,9 @@ def _declare(lib):
                                            I64, c_f32, c_vp, c_vp]),
         "ff_occupancy_pool": (ctypes.c_int, [c_vp, c_i32, c_i32, c_i32, c_f32, c_f32, c_vp, c_vp]),
         "ff_proposal_sample": (ctypes.c_int, [P(ProposalDesc), c_vp, c_vp, c_vp, I64, P(Samples), c_vp]),
+        "ff_proposal_bwd_workspace_bytes": (ctypes.c_size_t, [I64]),
         "ff_proposal_bwd": (ctypes.c_int, [P(ProposalDesc), c_vp, c_vp, I64, P(Samples), c_vp, c_f32,
-                                           P(ProposalGrads), c_vp, c_vp]),
+                                           P(ProposalGrads), c_vp, c_vp, ctypes.c_size_t, c_vp]),
         "ff_composite_fwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp]),
         "ff_composite_bwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,

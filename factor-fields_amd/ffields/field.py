@@ -255,7 +255,11 @@ class FactorFields(RenderMixin, nn.Module):
         return nn.functional.relu(density_features + r.density_shift)
 
     # ------------------------------------------------ fused training steps
-    grad_replicas = 8   # one gradient copy per XCD for the fused training steps
+    # Gradient copies the fused training steps add into (one per XCD when 8).
+    # Table-gradient atomics are bounded by the chip-wide cache-line request
+    # rate, which copies do not raise, while every extra copy costs the
+    # optimizer a read and a clear of all table gradients: one copy by default.
+    grad_replicas = 1
 
     def _ensure_grads(self):
         for p in self.parameters():

@@ -32,6 +32,7 @@ class ProposalNet(nn.Module):
         self.near = float(near)
         self.table_dtype = table_dtype
         self._shadow = None
+        self._ws = None
         if table_dtype == "bf16":
             self.refresh_shadow()
 
@@ -78,7 +79,11 @@ class ProposalNet(nn.Module):
         g.w1 = self.mlp[1].weight.grad.data_ptr()
         g.replicas, g.replica_stride = 1, 0
         d = self.desc()
+        nb = L.lib().ff_proposal_bwd_workspace_bytes(rays.shape[0])
+        ws = self._ws
+        if ws is None or ws.numel() * 4 < nb:
+            ws = self._ws = torch.empty((max(nb, 4) + 3) // 4, device=rays.device)
         L.check(L.lib().ff_proposal_bwd(ctypes.byref(d), L.ptr(rays), L.ptr(prop_u), rays.shape[0],
                                         ctypes.byref(samples_struct), L.ptr(weights), float(loss_scale),
-                                        ctypes.byref(g), L.ptr(loss_sum), L.stream_of(rays)),
+                                        ctypes.byref(g), L.ptr(loss_sum), L.ptr(ws), nb, L.stream_of(rays)),
                 "ff_proposal_bwd")

@@ -333,7 +333,10 @@ int ff_proposal_sample(const ff_proposal_desc* p, const float* rays, const float
  * loss_k = max(0, w_k - bound_k)^2 / (w_k + 1e-7), averaged over rays and
  * scaled by loss_scale.  Accumulates grid / MLP gradients (grid_grad with
  * `replicas` per-XCD copies spaced `replica_stride` floats; w0/b0/w1 grads
- * FP32) and loss_sum. */
+ * FP32, summed deterministically through the workspace) and loss_sum.
+ * ws must hold ff_proposal_bwd_workspace_bytes(n_rays) bytes
+ * (FF_ERR_WORKSPACE otherwise).  No counterpart in the reference (its
+ * training loop has no proposal network). */
 typedef struct ff_proposal_grads {
   float* grid;
   float* w0;
@@ -342,9 +345,11 @@ typedef struct ff_proposal_grads {
   int32_t replicas;
   int64_t replica_stride;
 } ff_proposal_grads;
+size_t ff_proposal_bwd_workspace_bytes(int64_t n_rays);
 int ff_proposal_bwd(const ff_proposal_desc* p, const float* rays, const float* prop_u,
                     int64_t n_rays, const ff_samples* fine, const float* weights, float loss_scale,
-                    const ff_proposal_grads* grads, float* loss_sum, void* stream);
+                    const ff_proposal_grads* grads, float* loss_sum, void* ws, size_t ws_bytes,
+                    void* stream);
 
 #ifdef __cplusplus
 }

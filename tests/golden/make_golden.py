@@ -5,7 +5,8 @@ configuration as a JSON string, and the expected outputs:
 
 * ``field_*``   -- get_coding / linear_mat outputs of small fields
                    (models/FactorFields.py:425-533, :113-159), computed in
-                   FP64 from FP32-valued tables and points;
+                   FP64 from FP32-valued tables and points, and in FP32 with
+                   the reference's operations (suffix 32);
 * ``mapping``   -- grid_mapping for every mapping kind
                    (models/FactorFields.py:11-33), FP64;
 * ``render``    -- the bounded-scene render (models/FactorFields.py:843-898)
@@ -119,11 +120,14 @@ def make_field(name, kw):
     cfg = small_cfg(**kw)
     of, tabs = seeded_field(cfg, seed=11)
     x = field_points(of, 256, seed=12)
+    of32, _ = seeded_field(cfg, seed=11, dtype=torch.float32)
     with torch.no_grad():
         feats, coeff = of.get_coding(x.double())
         out = of.linear_mat(feats)
+        feats32, coeff32 = of32.get_coding(x)
+        out32 = of32.linear_mat(feats32)
     return dict(cfg=np.array(json.dumps(cfg)), x=x.numpy(), feats=feats.numpy(), coeff=coeff.numpy(),
-                out=out.numpy(), **tabs)
+                out=out.numpy(), feats32=feats32.numpy(), coeff32=coeff32.numpy(), out32=out32.numpy(), **tabs)
 
 
 def make_mapping():

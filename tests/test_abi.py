@@ -88,3 +88,31 @@ def test_fused_support_query():
     m = L.MlpDesc()
     m.n_layers, m.in_dim, m.hidden, m.out_dim, m.w0, m.b0, m.w1 = 2, 4, 64, 3, 0x10, 0x20, 0x30
     assert L.lib().ff_field_fused_supported(ctypes.byref(d), ctypes.byref(m)) == 0   # 4 channels: generic
+
+
+def test_proposal_bwd_workspace_checked():
+    lib = L.lib()
+    nb = lib.ff_proposal_bwd_workspace_bytes(4096)
+    assert nb >= (4096 // 16 + 32) * 320 * 4          # at least one partial row per wave plus scratch
+    assert lib.ff_proposal_bwd_workspace_bytes(0) == 0
+    p = L.ProposalDesc()
+    p.reso[0] = p.reso[1] = p.reso[2] = 8
+    p.table_dtype = L.F32
+    p.grid = p.w0 = p.b0 = p.w1 = 0x1000
+    for k in range(3):
+        p.aabb_min[k], p.aabb_max[k] = -1.0, 1.0
+    p.n_fine = 48
+    s = L.Samples()
+    s.xyz = s.z = s.dist = s.ray = s.idx = s.ray_off = 0x2000
+    s.cap = 48 * 16
+    g = L.ProposalGrads()
+    g.grid = g.w0 = g.b0 = g.w1 = 0x3000
+    g.replicas = 1
+    st = lib.ff_proposal_bwd(ctypes.byref(p), ctypes.c_void_p(0x4000), None, 16, ctypes.byref(s),
+                             ctypes.c_void_p(0x5000), 1.0, ctypes.byref(g), None, ctypes.c_void_p(0x6000), 16, None)
+    assert st == L.FF_ERR_WORKSPACE
+    p.n_fine = 65
+    st = lib.ff_proposal_bwd(ctypes.byref(p), ctypes.c_void_p(0x4000), None, 16, ctypes.byref(s),
+                             ctypes.c_void_p(0x5000), 1.0, ctypes.byref(g), None, ctypes.c_void_p(0x6000), 1 << 30,
+                             None)
+    assert st == L.FF_ERR_ARG

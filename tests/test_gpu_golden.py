@@ -1,7 +1,8 @@
 """HIP path against the committed golden fixtures (tests/golden/*.npz).
 
-Bars: field features/outputs within 1e-5 relative of the FP64 fixture
-(absolute floor 1e-6 of the max); render sample counts and candidate indices
+Bars: field features/outputs within 1e-5 relative of the FP32 oracle fixture
+(absolute floor 1e-6 of the max) and within 1e-4 relative (floor 2e-5 of the
+max) of the FP64 one; render sample counts and candidate indices
 bit-exact, sigma/rgb_map/depth within 1e-5 relative; proposal fine samples
 within 1e-5 of the ray's span except at most 1% of them (an FP32 rounding
 difference at a CDF step moves a sample to the neighbouring bin)."""
@@ -54,9 +55,18 @@ def test_field_matches_fixture(name):
     with torch.no_grad():
         feats, coeff = m.get_coding(x)
         out = m.linear_mat(feats)
-    assert_close(feats, torch.from_numpy(fx["feats"]), 1e-5, "feats")
-    assert_close(coeff, torch.from_numpy(fx["coeff"]), 1e-5, "coeff")
-    assert_close(out, torch.from_numpy(fx["out"]), 1e-5, "out")
+    # the reference's FP32 semantics: 1e-5 relative
+    assert_close(feats, torch.from_numpy(fx["feats32"]), 1e-5, "feats")
+    assert_close(coeff, torch.from_numpy(fx["coeff32"]), 1e-5, "coeff")
+    assert_close(out, torch.from_numpy(fx["out32"]), 1e-5, "out")
+    # against the FP64 run: FP32 rounding of the coordinate mapping is amplified
+    # by the table slope, so the bar is 1e-4 relative with a floor of 2e-5 of
+    # the largest magnitude
+    for got, key in ((feats, "feats"), (coeff, "coeff"), (out, "out")):
+        ref = torch.from_numpy(fx[key])
+        err = (got.double().cpu() - ref).abs().max().item() / ref.abs().max().item()
+        print(f"[fp64] {name} {key}: max error / max |value| = {err:.2e}", file=sys.stderr, flush=True)
+        assert_close(got, ref, 1e-4, key + " vs fp64", atol_frac=2e-5)
 
 
 def test_render_matches_fixture():
@@ -64,14 +74,15 @@ def test_render_matches_fixture():
     fx = load("render")
     m = model_from(fx)
     m.alphaMask = AlphaGridMask(torch.device("cuda"), torch.from_numpy(fx["occ_aabb"]), torch.from_numpy(fx["occ"]))
-    assert m._step == float(fx["step"]) and m.nSamples == int(fx["n_samples"])
+    assert m._step == float(fx["step"])
     rays = torch.from_numpy(fx["rays"]).cuda()
     jit = torch.from_numpy(fx["jitter"]).cuda()
-    B, Ns = rays.shape[0], m.nSamples
+    # the sample count the training script passes (cal_n_samples, utils.py:69-70)
+    B, Ns = rays.shape[0], int(fx["n_samples"])
     buf = m.render_buffers(B, Ns)
     m.render_samples(rays, Ns, jit, buf)
     with torch.no_grad():
-        rgb, depth, _ = m(rays, white_bg=True, is_train=False, jitter=jit)
+        rgb, depth, _ = m(rays, white_bg=True, is_train=False, N_samples=Ns, jitter=jit)
     torch.cuda.synchronize()
     off = buf["ray_off"].cpu()
     M = int(off[-1])
