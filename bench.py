@@ -30,6 +30,32 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def copy_calibration():
+    """Three device-to-device copies of 1 GiB each: the reference dispatch for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters to HBM bytes (each
+    copy reads and writes exactly 2^30 bytes)."""
+    import torch
+    src = torch.empty(1 << 28, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured by the two rocprofv3 --pmc
+    runs recorded under profiles/ (null when no record exists)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    ent = rec.get("kernels", {}).get(kernel)
+    return None if ent is None else ent.get("hbm_bytes_per_launch")
+
+
 def sdf_bytes_per_point(model):
     """Algorithmic HBM bytes per point of the fused SDF training kernel:
     coordinates (12 B) + target (4 B) + one read of every gathered table value
@@ -242,9 +268,9 @@ def run_nerf(args):
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     log("nerf: ground-truth ray pool")
-    _, pool, pool_rgb = nerf_setup(dev)
+    _, pool, pool_rgb = nerf_setup(dev, pool_rays=args.nerf_pool)
     _, test_rays, test_rgb = nerf_setup(dev, pool_rays=1 << 14, seed_off=7)
-    forms = ["plain"] if args.nerf_form == "plain" else ["plain", "proposal"]
+    forms = {"both": ["plain", "proposal"], "plain": ["plain"], "proposal": ["proposal"]}[args.nerf_form]
     res = {}
     for f in forms:
         log(f"nerf {f}: {args.nerf_pretrain} training steps, then {args.steps} timed")
@@ -274,9 +300,10 @@ def run_nerf(args):
         out["config"]["speedup_vs_plain_fp32"] = res["proposal"]["rays_per_s"] / res["plain"]["rays_per_s"]
         out["config"]["psnr_gap_db"] = res["plain"]["test_psnr"] - res["proposal"]["test_psnr"]
     bw = main["field_bwd_gbs"]
+    kname = "nerf_field_bwd_mfma_kernel" if forms[-1] == "proposal" else "nerf_field_bwd_kernel"
     out["roofline"] = {"bound": "hbm", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": bw / HBM_PEAK_GBS if bw else None, "traffic": None,
-                       "kernel": "nerf_field_bwd_kernel (+ its decoder-gradient reduction)",
+                       "frac": bw / HBM_PEAK_GBS if bw else None, "traffic": measured_traffic(kname),
+                       "kernel": kname + " (+ its decoder-gradient reduction)",
                        "kernel_ms": main["field_bwd_ms"], "bytes_per_sample": main["field_bwd_bytes_per_sample"]}
     return out
 
@@ -428,18 +455,23 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="nerf", choices=["sdf", "nerf", "image"])
-    ap.add_argument("--nerf-form", default="both", choices=["both", "plain"])
+    ap.add_argument("--nerf-form", default="both", choices=["both", "plain", "proposal"])
+    ap.add_argument("--nerf-pool", type=int, default=1 << 20, help="ground-truth rays the batches are drawn from")
     ap.add_argument("--image-iters", type=int, default=2000)
     ap.add_argument("--nerf-pretrain", type=int, default=300)
     ap.add_argument("--cpu-batch", type=int, default=1 << 15)
     ap.add_argument("--cpu-rays", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run three 1 GiB device copies first (reference dispatch for rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU and this build has no "
               "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
+    if args.copy_calibration:
+        copy_calibration()
     if args.workload == "nerf":
         result = run_nerf(args)
         if rank == 0 and not args.no_cpu_baseline:

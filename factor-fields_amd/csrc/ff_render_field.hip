@@ -5,8 +5,11 @@
 // linear_mat, basis2density :639-643) with the colour head
 // sigmoid(eval_sh(2, feat[1:28].view(3, 9), viewdir)) (models/sh.py:34-64).
 // The sample count is read on the device (ff_samples.ray_off[n_rays]); the
-// grid is persistent and every wave strides over 64-sample tiles.
+// grid is persistent and every wave strides over 64-sample tiles.  With BF16
+// tables the decoder runs on the matrix cores (BF16 operands, FP32
+// accumulation, ff_mfma.h); the FP32 form keeps the FP32 vector decoder.
 #include "ff_field_impl.h"
+#include "ff_mfma.h"
 
 namespace ff {
 
@@ -56,6 +59,23 @@ __device__ __forceinline__ float softplus_grad(float x) {
   if (x > 20.f) return 1.f;
   const float z = expf(x);
   return z / (z + 1.f);
+}
+
+// d out for one sample: softplus/ReLU density and sigmoid(SH) colour heads.
+__device__ __forceinline__ void nerf_dout(const ff_render_head& hd, const float* o, float4 gs, float dx, float dy,
+                                          float dz, float* dout) {
+  const float x0 = o[0] + hd.density_shift;
+  dout[0] = gs.x * (hd.density_act == 0 ? softplus_grad(x0) : (x0 > 0.f ? 1.f : 0.f));
+  float Y[9];
+  sh2_basis(dx, dy, dz, Y);
+  const float gc[3] = {gs.y, gs.z, gs.w};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float y = sigmoidf(sh2_eval(o + 1 + 9 * c, dx, dy, dz));
+    const float dpre = gc[c] * (1.f - y) * y;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dout[1 + 9 * c + k] = dpre * Y[k];
+  }
 }
 
 template <int CK, bool BF>
@@ -113,26 +133,112 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff
     combine<C, CK>(fd, bas, cof, f);
     mlp_train_tile_fn<C, 64, 28, 2>(
         m, acc, lds, f, active,
-        [&](const float* o, float* dout) {
-          const float x0 = o[0] + hd.density_shift;
-          dout[0] = gs.x * (hd.density_act == 0 ? softplus_grad(x0) : (x0 > 0.f ? 1.f : 0.f));
-          float Y[9];
-          sh2_basis(dx, dy, dz, Y);
-          const float gc[3] = {gs.y, gs.z, gs.w};
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const float y = sigmoidf(sh2_eval(o + 1 + 9 * c, dx, dy, dz));
-            const float dpre = gc[c] * (1.f - y) * y;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) dout[1 + 9 * c + k] = dpre * Y[k];
-          }
-        },
+        [&](const float* o, float* dout) { nerf_dout(hd, o, gs, dx, dy, dz, dout); },
         df);
     float db[C], dc[C];
     combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
     coding_bwd_wave<3, CK, BF, Spec18>(fd, gl, xv, db, dc, active, lds);
   }
   mlp_acc_store<C, 64, 28, 2>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
+}
+
+// BF16-table forward with the matrix-core decoder: one wave per 64-sample tile.
+template <int CK>
+__global__ __launch_bounds__(256) void nerf_field_fwd_mfma_kernel(ff_field_desc fd, ff_mlp_desc m,
+                                                                  ff_render_head hd,
+                                                                  const float* __restrict__ rays,
+                                                                  ff_samples s, int64_t n_rays,
+                                                                  float* __restrict__ sigma,
+                                                                  float* __restrict__ rgb) {
+  constexpr int C = Spec18::C;
+  MfmaDecoder W;
+  W.load(m);
+  const int lane = threadIdx.x & 63;
+  const int64_t n = s.ray_off[n_rays];
+  const int64_t ntiles = (n + kWave - 1) / kWave;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
+    const int64_t i = tile * kWave + lane;
+    const bool active = i < n;
+    const int64_t ic = active ? i : n - 1;
+    float xv[3] = {s.xyz[ic * 3 + 0], s.xyz[ic * 3 + 1], s.xyz[ic * 3 + 2]};
+    float bas[C], cof[C], f[C], o[32];
+    coding_fwd<3, CK, true, Spec18>(fd, xv, bas, cof);
+    combine<C, CK>(fd, bas, cof, f);
+    mfma_decoder_fwd(W, f, o);
+    if (active) {
+      const int64_t rb = s.ray[i];
+      const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
+      const float x0 = o[0] + hd.density_shift;
+      sigma[i] = hd.density_act == 0 ? softplusf(x0) : fmaxf(x0, 0.f);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = sigmoidf(sh2_eval(o + 1 + 9 * c, dx, dy, dz));
+    }
+  }
+}
+
+// BF16-table backward: gathers recomputed, matrix-core decoder forward and
+// backward, run-merging table scatter; decoder weight gradients accumulate in
+// the matrix-core accumulators across the wave's tiles.
+template <int CK>
+__global__ __launch_bounds__(64) void nerf_field_bwd_mfma_kernel(ff_field_desc fd, ff_mlp_desc m,
+                                                                 ff_render_head hd, ff_grads g,
+                                                                 const float* __restrict__ rays,
+                                                                 ff_samples s, int64_t n_rays,
+                                                                 const float* __restrict__ d_sample,
+                                                                 float* __restrict__ ws) {
+  constexpr int C = Spec18::C;
+  // LDS: decoder operands (later the scatter staging area) | gathered values
+  constexpr int kDec = kMfLdsWords > kStageWords ? kMfLdsWords : kStageWords;
+  constexpr int kValRow = 2 * C + 1;
+  __shared__ __attribute__((aligned(16))) uint32_t lds[kDec + kWave * kValRow];
+  float* vals = reinterpret_cast<float*>(lds + kDec) + threadIdx.x * kValRow;
+  MfmaDecoder W;
+  W.load(m);
+  MfmaDecoderBwd WB;
+  WB.load(m);
+  MfmaDecoderAcc acc;
+  acc.zero();
+  const int lane = threadIdx.x;
+  const int64_t n = s.ray_off[n_rays];
+  const int64_t ntiles = (n + kWave - 1) / kWave;
+  const ff_grads gl = local_grads(g);
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t i = tile * kWave + lane;
+    const bool active = i < n;
+    const int64_t ic = active ? i : n - 1;
+    float xv[3] = {s.xyz[ic * 3 + 0], s.xyz[ic * 3 + 1], s.xyz[ic * 3 + 2]};
+    float dout[32], df[C];
+    uint32_t msk[2][2];
+    {
+      float bas[C], cof[C], f[C], o[32];
+      coding_fwd<3, CK, true, Spec18>(fd, xv, bas, cof);
+      combine<C, CK>(fd, bas, cof, f);
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        vals[k] = bas[k];
+        vals[C + k] = cof[k];
+      }
+      mfma_decoder_fwd_train(W, f, o, msk, lds);
+      const int64_t rb = s.ray[ic];
+      const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
+      const float4 gs = reinterpret_cast<const float4*>(d_sample)[ic];
+      nerf_dout(hd, o, gs, dx, dy, dz, dout);
+#pragma unroll
+      for (int k = 0; k < 32; ++k) dout[k] = (active && k < kMfO) ? dout[k] : 0.f;
+    }
+    mfma_decoder_bwd(WB, msk, dout, df, acc, lds);
+    float bas[C], cof[C], db[C], dc[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      bas[k] = vals[k];
+      cof[k] = vals[C + k];
+    }
+    combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
+    coding_bwd_wave<3, CK, true, Spec18>(fd, gl, xv, db, dc, active, reinterpret_cast<float*>(lds));
+  }
+  acc.store(ws + (int64_t)blockIdx.x * (kMfH * kMfIn + kMfH + kMfO * kMfH));
 }
 
 // compute_alpha (models/FactorFields.py:710-727) accumulated into alpha.
@@ -235,8 +341,12 @@ int ff_render_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
   return with_nerf(d, [&](auto CKi, auto BFi) {
     constexpr int CK = decltype(CKi)::value;
     constexpr bool BF = decltype(BFi)::value;
-    hipLaunchKernelGGL((nerf_field_fwd_kernel<CK, BF>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
-                       n_rays, sigma, rgb);
+    if constexpr (BF)
+      hipLaunchKernelGGL((nerf_field_fwd_mfma_kernel<CK>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
+                         n_rays, sigma, rgb);
+    else
+      hipLaunchKernelGGL((nerf_field_fwd_kernel<CK, BF>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
+                         n_rays, sigma, rgb);
     return rf_status();
   });
 }
@@ -297,8 +407,12 @@ int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
   int r = with_nerf(d, [&](auto CKi, auto BFi) {
     constexpr int CK = decltype(CKi)::value;
     constexpr bool BF = decltype(BFi)::value;
-    hipLaunchKernelGGL((nerf_field_bwd_kernel<CK, BF>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays, sv,
-                       n_rays, d_sample, (float*)ws);
+    if constexpr (BF)
+      hipLaunchKernelGGL((nerf_field_bwd_mfma_kernel<CK>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays, sv,
+                         n_rays, d_sample, (float*)ws);
+    else
+      hipLaunchKernelGGL((nerf_field_bwd_kernel<CK, BF>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays, sv,
+                         n_rays, d_sample, (float*)ws);
     return rf_status();
   });
   if (r) return r;

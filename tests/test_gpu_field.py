@@ -28,6 +28,11 @@ CASES = {
     "3d_x_zeros_padding": dict(mode="sdf", D=3, ctype="grid", mapping="x"),
     "generic_dims_2d": dict(mode="image", D=2, ctype="grid", dims=(3, 5, 2), resos=(7, 9, 4)),
     "generic_dims_vm": dict(mode="sdf", D=3, ctype="vm", dims=(2, 3), resos=(6, 8)),
+    # image-set mode (models/FactorFields.py:286-288, :469-470): 2-D basis on (x, y),
+    # 3-D coefficient grid [C, n_images, h, w] sampled at (x, y, image)
+    "images_grid": dict(mode="images", D=2, ctype="grid", aabb=[[0.0, 0.0, 0.0], [8.0, 6.0, 5.0]]),
+    "images_generic_dims": dict(mode="images", D=2, ctype="grid", dims=(6, 3), resos=(7, 9),
+                                aabb=[[0.0, 0.0, 0.0], [8.0, 6.0, 4.0]]),
 }
 
 
@@ -139,10 +144,20 @@ def test_regression_step(name, loss):
         assert_close(lin.weight.grad, w.grad, GRAD_RTOL, f"W{l}", atol_frac=1e-5)
 
 
-def test_bf16_tables_forward():
+BF16_CASES = {
+    "3d_grid": dict(mode="sdf", D=3, ctype="grid"),
+    # the dictionary image-set layout: 144 channels, image-set coefficient grid
+    "images_dict_c144": dict(mode="images", D=2, ctype="grid", dims=(32, 32, 32, 16, 16, 16),
+                             resos=(5, 6, 7, 8, 9, 10), coeff_reso=4, aabb=[[0.0, 0.0, 0.0], [8.0, 6.0, 3.0]]),
+}
+
+
+@pytest.mark.parametrize("name", list(BF16_CASES))
+def test_bf16_tables_forward(name):
     """BF16 tables: same result as the oracle evaluated on the BF16-rounded
     tables (FP32 arithmetic)."""
-    cfg = small_cfg(mode="sdf", D=3, ctype="grid")
+    print(f"[case] bf16 {name}", file=sys.stderr, flush=True)
+    cfg = small_cfg(**BF16_CASES[name])
     m = new_model(cfg, dtype="bf16")
     of = oracle_of(m, bf16_tables=True)
     x = rand_points(m, 4000, seed=12)
