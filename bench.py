@@ -296,7 +296,7 @@ def run_nerf(args):
                      "18-64-28 decoder + degree-2 SH head",
                    "field_pts_per_s": main["field_pts_per_s"], "forms": res},
     }
-    if "proposal" in res:
+    if "proposal" in res and "plain" in res:
         out["config"]["speedup_vs_plain_fp32"] = res["proposal"]["rays_per_s"] / res["plain"]["rays_per_s"]
         out["config"]["psnr_gap_db"] = res["plain"]["test_psnr"] - res["proposal"]["test_psnr"]
     bw = main["field_bwd_gbs"]
@@ -458,7 +458,7 @@ def main():
     ap.add_argument("--nerf-form", default="both", choices=["both", "plain", "proposal"])
     ap.add_argument("--nerf-pool", type=int, default=1 << 20, help="ground-truth rays the batches are drawn from")
     ap.add_argument("--image-iters", type=int, default=2000)
-    ap.add_argument("--nerf-pretrain", type=int, default=300)
+    ap.add_argument("--nerf-pretrain", type=int, default=1000)
     ap.add_argument("--cpu-batch", type=int, default=1 << 15)
     ap.add_argument("--cpu-rays", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")

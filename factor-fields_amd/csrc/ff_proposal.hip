@@ -19,6 +19,7 @@
 namespace ff {
 
 int launch_scan_offsets(int32_t* off, int64_t B, hipStream_t s);  // ff_render.hip
+int launch_zero_i32(int32_t* p, int n, hipStream_t s);             // ff_render.hip
 
 typedef const __attribute__((address_space(4))) float* pcfp;
 constexpr int kPH = 32;  // proposal MLP hidden width
@@ -362,10 +363,7 @@ int ff_proposal_sample(const ff_proposal_desc* p, const float* rays, const float
   if (!out->xyz || !out->z || !out->dist || !out->ray || !out->idx) return FF_ERR_ARG;
   if (out->cap < n_rays * (int64_t)p->n_fine) return FF_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  if (n_rays == 0) {
-    hipMemsetAsync(out->ray_off, 0, sizeof(int32_t), s);
-    return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
-  }
+  if (n_rays == 0) return launch_zero_i32(out->ray_off, 1, s);
   const unsigned blocks = (unsigned)((n_rays * 64 + 255) / 256);
   const ff_proposal_desc pd = *p;
   auto go = [&](auto BFi) {

@@ -343,6 +343,19 @@ __global__ __launch_bounds__(256) void occupancy_pool_kernel(const float* __rest
   out[i] = m >= thres ? 1 : 0;
 }
 
+// Zero a small int32 range with a kernel (not a memset node, so that a caller
+// may capture the entry point in a HIP graph).
+__global__ void zero_i32_kernel(int32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+int launch_zero_i32(int32_t* p, int n, hipStream_t s) {
+  if (n <= 0) return FF_OK;
+  hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
+}
+
 int launch_scan_offsets(int32_t* off, int64_t B, hipStream_t s) {
   hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(1024), 0, s, off, B);
   return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
@@ -367,10 +380,7 @@ int ff_sample_rays(const ff_ray_sampler* sampler, const float* rays, const float
     for (int a = 0; a < 3; ++a)
       if (s.occ_reso[a] < 2) return FF_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
-  if (n_rays == 0) {
-    hipMemsetAsync(out->ray_off, 0, sizeof(int32_t), st);
-    return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
-  }
+  if (n_rays == 0) return launch_zero_i32(out->ray_off, 1, st);
   const unsigned blocks = (unsigned)((n_rays * 64 + 255) / 256);
   hipLaunchKernelGGL(sample_rays_kernel, dim3(blocks), dim3(256), 0, st, s, rays, jitter, n_rays, 0,
                      *out);

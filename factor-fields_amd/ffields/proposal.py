@@ -25,6 +25,7 @@ class ProposalNet(nn.Module):
         self.grid = nn.Parameter(texel_major(grid.to(self.device)))
         self.mlp = nn.ModuleList([nn.Linear(8, hidden), nn.Linear(hidden, 1, bias=False)]).to(self.device)
         self.aabb = torch.as_tensor(aabb, dtype=torch.float32).to(self.device)
+        self._aabb_list = torch.as_tensor(aabb, dtype=torch.float32).cpu().tolist()
         self.reso = reso
         self.n_fine = n_fine
         self.density_shift = float(density_shift)
@@ -53,7 +54,7 @@ class ProposalNet(nn.Module):
         d.w0 = self.mlp[0].weight.data_ptr()
         d.b0 = self.mlp[0].bias.data_ptr()
         d.w1 = self.mlp[1].weight.data_ptr()
-        a = self.aabb.cpu().tolist()
+        a = self._aabb_list
         for k in range(3):
             d.aabb_min[k], d.aabb_max[k] = a[0][k], a[1][k]
         d.density_shift = self.density_shift

@@ -9,6 +9,7 @@ kernel, so ``renderModule`` holds no parameters.
 """
 
 import ctypes
+import warnings
 
 import numpy as np
 import torch
@@ -94,9 +95,19 @@ class RenderMixin:
         self.nSamples = int((aabbDiag / self.stepSize).item()) + 1
         self._step = float(self.stepSize.item())
 
+    def _aabb_host(self):
+        """Host copy of the box (the kernels take it by value); refreshed only
+        when the device tensor object changes, so a training step does not
+        synchronise with the device."""
+        ref = getattr(self, "_aabb_ref", None)
+        if ref is None or ref[0] is not self.aabb or ref[1] != self.aabb._version:
+            self._aabb_list = self.aabb.cpu().tolist()
+            self._aabb_ref = (self.aabb, self.aabb._version)   # holds the tensor: no id reuse
+        return self._aabb_list
+
     def _sampler(self, n_samples):
         s = L.RaySampler()
-        a = self.aabb.cpu().tolist()
+        a = self._aabb_host()
         for k in range(3):
             s.aabb_min[k], s.aabb_max[k] = a[0][k], a[1][k]
         s.step_size = self._step
@@ -364,6 +375,11 @@ class RenderMixin:
         L.check(L.lib().ff_occupancy_pool(L.ptr(alpha), Z, Y, X, 1.0, thres, L.ptr(occ),
                                           L.stream_of(alpha)), "ff_occupancy_pool")
         keep = _remove_small_objects(occ.cpu().numpy().astype(bool), min_size=float(np.mean([Z, Y, X])))
+        if not keep.any():
+            # Nothing above the threshold yet (an untrained field): the reference
+            # would fail on the empty reduction below; keep the current mask and box.
+            warnings.warn("updateAlphaMask: no voxel above the threshold; mask and box unchanged")
+            return self.aabb.clone()
         alpha = torch.from_numpy(keep.astype(np.float32)).to(self.device)
         if is_update_alphaMask:
             self.alphaMask = AlphaGridMask(self.device, self.inward_aabb, alpha)
