@@ -177,6 +177,7 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
     from ffields import FactorFields, FusedAdam
     from ffields import workloads as W
     from ffields.params import N_to_reso, cal_n_samples
+    from ffields.proposal import ProposalReferee
 
     cfg = W.nerf_cfg()
     torch.manual_seed(W.BASE_SEED)
@@ -192,18 +193,19 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
     lr_factor = 0.1 ** (1 / cfg.training.n_iters)
     gen = torch.Generator(device=dev).manual_seed(W.BASE_SEED + 2001)
     counts = torch.zeros(max(timed_steps, 1), dtype=torch.int32, device=dev)
-    sq_err = torch.zeros(2, device=dev)       # squared error of: occupancy sampler, proposal
-    state = {"active": prop, "fallback_at": None, "gaps": []}
+    referee = ProposalReferee(dev, n_rays=check_rays, max_gap_db=0.1, n_samples=n_samples) if prop else None
+    state = {"fallback_at": None}
+
+    def proposal_on():
+        return referee is not None and referee.active
 
     def step(k=None):
         idx = torch.randint(0, pool.shape[0], (B,), device=dev, generator=gen)
         rays, rgb = pool[idx], pool_rgb[idx]
-        if state["active"]:
+        if proposal_on():
+            referee.compare(model, rays, rgb)          # side stream, overlaps the backward
             model.render_train_step_proposal(rays, rgb, white_bg=True, reduce=False)
-            if model.alphaMask is not None:
-                jr, jt = rays[:check_rays], rgb[:check_rays]
-                sq_err[0] += ((model.render_eval(jr, use_proposal=False, N_samples=n_samples) - jt) ** 2).sum()
-                sq_err[1] += ((model.render_eval(jr, use_proposal=True) - jt) ** 2).sum()
+            referee.join()
         else:
             model.render_train_step(rays, rgb, white_bg=True, N_samples=n_samples, reduce=False)
         if k is not None:
@@ -212,25 +214,18 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
         for gp in opt.param_groups:
             gp["lr"] *= lr_factor
 
-    def compare_samplers(it):
-        s0, s1 = sq_err.tolist()
-        sq_err.zero_()
-        if s0 > 0 and s1 > 0:
-            gap = 10 * math.log10(s1 / s0)       # PSNR(occupancy sampler) - PSNR(proposal), dB
-            state["gaps"].append(round(gap, 4))
-            if gap > 0.1:
-                state["active"] = False
-                state["fallback_at"] = it
+    def referee_check(it):
+        if proposal_on() and referee.decide() is not None and not referee.active:
+            state["fallback_at"] = it
 
     upd = pretrain // 2
     for it in range(pretrain):
         step()
         if it == upd:
             model.updateAlphaMask(tuple(reso), is_update_alphaMask=True)
-        if state["active"] and it > upd and (it - upd) % 100 == 0:
-            compare_samplers(it)
-    if state["active"] and model.alphaMask is not None:
-        compare_samplers(pretrain)
+        if it > upd and (it - upd) % 100 == 0:
+            referee_check(it)
+    referee_check(pretrain)
     occ_frac = float(model.alphaMask.occ.float().mean()) if model.alphaMask is not None else 1.0
     torch.cuda.synchronize()
     model.kernel_events = []
@@ -244,15 +239,16 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
     kern_ms = sum(a.elapsed_time(b) for a, b in kern) / max(len(kern), 1)
     samples = int(counts[:timed_steps].sum())
     with torch.no_grad():
-        out = [model.render_eval(test_rays[k:k + 4096], use_proposal=state["active"], N_samples=n_samples)
+        out = [model.render_eval(test_rays[k:k + 4096], use_proposal=proposal_on(), N_samples=n_samples)
                for k in range(0, test_rays.shape[0], 4096)]
         mse = torch.mean((torch.cat(out) - test_rgb) ** 2).item()
     bps, G = nerf_bytes_per_sample(model, 2 if prop else 4)
     samples_per_launch = samples / max(timed_steps, 1)
     return {"rays_per_s": timed_steps * B / dt, "field_pts_per_s": samples / dt,
             "ms_per_step": dt / timed_steps * 1e3, "samples_per_ray": samples / (timed_steps * B),
-            "test_psnr": -10 * math.log10(mse), "proposal_active": state["active"],
-            "fallback_at_step": state["fallback_at"], "sampler_gaps_db": state["gaps"][-5:],
+            "test_psnr": -10 * math.log10(mse), "proposal_active": proposal_on(),
+            "fallback_at_step": state["fallback_at"],
+            "sampler_gaps_db": referee.gaps_db[-5:] if referee is not None else [],
             "train_steps": pretrain + timed_steps, "occupancy_fraction": occ_frac, "n_samples": n_samples,
             "table_dtype": "bf16" if prop else "fp32", "params": model.n_parameters(),
             "field_bwd_ms": kern_ms, "field_bwd_bytes_per_sample": bps, "gathered_values_per_sample": G,

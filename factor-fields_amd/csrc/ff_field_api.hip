@@ -169,9 +169,12 @@ int with_mlp(const ff_mlp_desc& m, int in_dim, F&& f) {
 
 int partial_floats(int NL, int IN, int H, int O) { return NL == 1 ? O * IN : H * IN + H + O * H; }
 
+// The fused backward kernels run one 64-thread block per SIMD (register- and
+// LDS-limited), so four blocks per CU are all that is resident; more blocks
+// would only add decoder-gradient partial rows to reduce.
 unsigned persistent_blocks(int64_t n) {
   const int64_t tiles = (n + kWaveSize - 1) / kWaveSize;
-  const int64_t cap = (int64_t)cu_count() * 16;
+  const int64_t cap = (int64_t)cu_count() * 4;
   return (unsigned)(tiles < cap ? tiles : cap);
 }
 

@@ -69,38 +69,6 @@ __device__ __forceinline__ void basis_level_fwd(const ff_field_desc& fd, int l, 
   }
 }
 
-template <int D, int dl>
-__device__ __forceinline__ void basis_level_bwd(const ff_field_desc& fd, const ff_grads& g, int l,
-                                                const float* x, const float* grad) {
-  const int R = fd.basis_reso[l];
-  float pos[D];
-  int Rs[D];
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    pos[a] = unnorm_ac(map_coord(fd.mapping, x[a], fd.map_origin[a], fd.map_scale[l]), R);
-    Rs[a] = R;
-  }
-  float* dst = g.basis[l];
-  if (fd.basis_interp == FF_INTERP_BILINEAR) {
-    Corners<D> cr;
-    linear_corners<D>(pos, Rs, cr);
-#pragma unroll
-    for (int k = 0; k < (1 << D); ++k) {
-      if (cr.w[k] != 0.f) {
-        float* p = dst + (int64_t)cr.tex[k] * dl;
-#pragma unroll
-        for (int c = 0; c < dl; ++c) unsafeAtomicAdd(p + c, cr.w[k] * grad[c]);
-      }
-    }
-  } else {
-    const int t = nearest_texel<D>(pos, Rs);
-    if (t >= 0) {
-      float* p = dst + (int64_t)t * dl;
-#pragma unroll
-      for (int c = 0; c < dl; ++c) unsafeAtomicAdd(p + c, grad[c]);
-    }
-  }
-}
 
 // Normalised coefficient coordinate (models/FactorFields.py:635-637).
 template <int XD>
@@ -136,32 +104,6 @@ __device__ __forceinline__ void grid_sample_border(const void* tab, const int* R
   }
 }
 
-template <int ND, int C>
-__device__ __forceinline__ void grid_scatter_border(float* dst, const int* R, const float* pts,
-                                                    int interp, const float* grad) {
-  float pos[ND];
-#pragma unroll
-  for (int a = 0; a < ND; ++a) pos[a] = unnorm_border(pts[a], R[a]);
-  if (interp == FF_INTERP_BILINEAR) {
-    Corners<ND> cr;
-    linear_corners<ND>(pos, R, cr);
-#pragma unroll
-    for (int k = 0; k < (1 << ND); ++k) {
-      if (cr.w[k] != 0.f) {
-        float* p = dst + (int64_t)cr.tex[k] * C;
-#pragma unroll
-        for (int c = 0; c < C; ++c) unsafeAtomicAdd(p + c, cr.w[k] * grad[c]);
-      }
-    }
-  } else {
-    const int t = nearest_texel<ND>(pos, R);
-    if (t >= 0) {
-      float* p = dst + (int64_t)t * C;
-#pragma unroll
-      for (int c = 0; c < C; ++c) unsafeAtomicAdd(p + c, grad[c]);
-    }
-  }
-}
 
 // VM plane m and line m values at pts (3-D).
 template <int C, bool BF>
@@ -209,44 +151,6 @@ __device__ __forceinline__ void coding_fwd(const ff_field_desc& fd, const float*
   }
 }
 
-// Scatter d(bas) and d(cof) into the table gradients.
-template <int D, int CK, bool BF, class S>
-__device__ __forceinline__ void coding_bwd(const ff_field_desc& fd, const ff_grads& g,
-                                           const float* x, const float* dbas, const float* dcof) {
-  constexpr int C = S::C;
-  if (fd.basis_kind == FF_BASIS_GRID) {
-    static_for<0, S::L>([&](auto LI) {
-      constexpr int l = decltype(LI)::value;
-      basis_level_bwd<D, S::dim(l)>(fd, g, l, x, dbas + S::off(l));
-    });
-  }
-  constexpr int XD = XDim<D, CK>::value;
-  if constexpr (CK == CK_GRID2 || CK == CK_GRID3) {
-    constexpr int ND = CK == CK_GRID2 ? 2 : 3;
-    float pts[XD];
-    coeff_pts<XD>(fd, x, pts);
-    grid_scatter_border<ND, C>(g.coeff[0], fd.coeff_reso, pts, fd.coeff_interp, dcof);
-  } else if constexpr (CK == CK_VM) {
-    float pts[3];
-    coeff_pts<3>(fd, x, pts);
-#pragma unroll 1
-    for (int m = 0; m < 3; ++m) {
-      float pv[C], lv[C], t[C];
-      vm_terms<C, BF>(fd, m, pts, pv, lv);
-      const int a0 = kMatMode[m][0], a1 = kMatMode[m][1], av = kVecMode[m];
-      const int Rp[2] = {fd.coeff_reso[a0], fd.coeff_reso[a1]};
-      const float pp[2] = {pts[a0], pts[a1]};
-#pragma unroll
-      for (int c = 0; c < C; ++c) t[c] = dcof[c] * lv[c];
-      grid_scatter_border<2, C>(g.coeff[m], Rp, pp, fd.coeff_interp, t);
-      const int Rl[2] = {1, fd.coeff_reso[av]};
-      const float pl[2] = {0.f, pts[av]};
-#pragma unroll
-      for (int c = 0; c < C; ++c) t[c] = dcof[c] * pv[c];
-      grid_scatter_border<2, C>(g.coeff[3 + m], Rl, pl, fd.coeff_interp, t);
-    }
-  }
-}
 
 // ------------------------------------------------------------ coalesced scatter
 // Atomic throughput is bounded by cache-line requests, not by the lanes each
@@ -255,10 +159,10 @@ __device__ __forceinline__ void coding_bwd(const ff_field_desc& fd, const ff_gra
 // (texel, weight) and gradient vector in a per-wave LDS area; for each corner
 // slot k, consecutive points whose corner k is the same texel (samples along
 // a ray, neighbouring query points) form a run, and only the first point of
-// each run enters a head list (wave prefix sum of per-point head counts).  Groups of d
-// consecutive lanes then take one head each, sum w * grad over its run in
-// registers and add the texel's d contiguous channels with one atomic per
-// channel.  Staging layout (words): G [64*d] | W [64*K] | T [64*K] |
+// each run enters a head list (wave prefix sum of per-point head counts).
+// Groups of d consecutive lanes then take one head each, sum w * grad over its
+// run in registers and add the texel's d contiguous channels with one atomic
+// per channel.  Staging layout (words): G [64*d] | W [64*K] | T [64*K] |
 // heads [64*K].  Every lane of the wave must call wave_scatter, with the same
 // K and d.
 constexpr int kStageWords = 64 * 18 + 64 * 8 * 3;

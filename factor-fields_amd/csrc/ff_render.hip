@@ -233,28 +233,35 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
     const float* __restrict__ target, float loss_scale, float* __restrict__ loss_sum,
     float* __restrict__ d_sample) {
   const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
   const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (b >= B) return;
-  const int64_t s0 = s.ray_off[b];
-  const int cnt = s.ray_off[b + 1] - (int)s0;
   // dL/d rgb_map, then through the clamp
   float g[3];
   float lsum = 0.f;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const float raw = rgb_raw[b * 3 + c];
+    const float raw = b < B ? rgb_raw[b * 3 + c] : 0.f;
     float gc;
     if (target) {
       const float mapped = fminf(fmaxf(raw, 0.f), 1.f);
-      const float e = mapped - target[b * 3 + c];
+      const float e = b < B ? mapped - target[b * 3 + c] : 0.f;
       gc = (2.f * e) * (loss_scale / (float)(3.0 * (double)B));
       lsum += e * e;
     } else {
-      gc = d_rgb_map[b * 3 + c];
+      gc = b < B ? d_rgb_map[b * 3 + c] : 0.f;
     }
     g[c] = (raw >= 0.f && raw <= 1.f) ? gc : 0.f;
   }
-  if (target && loss_sum && lane == 0) unsafeAtomicAdd(loss_sum, lsum);
+  // loss: one atomic per block (the rays' sums added in LDS), not one per ray
+  if (target && loss_sum) {
+    __shared__ float part[4];
+    if (lane == 0) part[wib] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) unsafeAtomicAdd(loss_sum, (part[0] + part[1]) + (part[2] + part[3]));
+  }
+  if (b >= B) return;
+  const int64_t s0 = s.ray_off[b];
+  const int cnt = s.ray_off[b + 1] - (int)s0;
   const float gbg = white_bg ? (g[0] + g[1]) + g[2] : 0.f;
   const int rounds = (cnt + 63) / 64;
   if (rounds > kMaxRounds) return;  // the sampler never produces more than 2048 per ray

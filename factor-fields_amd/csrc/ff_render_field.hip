@@ -151,13 +151,14 @@ __global__ __launch_bounds__(256) void nerf_field_fwd_mfma_kernel(ff_field_desc 
                                                                   float* __restrict__ sigma,
                                                                   float* __restrict__ rgb) {
   constexpr int C = Spec18::C;
-  MfmaDecoder W;
-  W.load(m);
   const int lane = threadIdx.x & 63;
   const int64_t n = s.ray_off[n_rays];
   const int64_t ntiles = (n + kWave - 1) / kWave;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  if (wave >= ntiles) return;  // the grid is sized for the capacity, not the live count
+  MfmaDecoder W;
+  W.load(m);
   for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
     const int64_t i = tile * kWave + lane;
     const bool active = i < n;
@@ -316,6 +317,9 @@ unsigned rf_blocks(int64_t cap, int per, int64_t per_cu) {
 }
 
 constexpr int kNerfPartial = 64 * 18 + 64 + 28 * 64;
+// Backward kernels: one wave per SIMD is resident (registers, LDS), so four
+// 64-thread blocks per CU; each block writes one decoder-gradient partial row.
+constexpr int kBwdBlocksPerCu = 4;
 
 }  // namespace
 }  // namespace ff
@@ -379,7 +383,7 @@ int ff_render_alpha(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_
 size_t ff_render_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp,
                                            int64_t cap) {
   if (!desc || !mlp || cap <= 0 || !nerf_supported(*desc, *mlp)) return 0;
-  return ((size_t)rf_blocks(cap, 64, 16) + kReduceSplit) * kNerfPartial * 4;
+  return ((size_t)rf_blocks(cap, 64, kBwdBlocksPerCu) + kReduceSplit) * kNerfPartial * 4;
 }
 
 int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
@@ -396,7 +400,7 @@ int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
     if (!grads->coeff[k]) return FF_ERR_ARG;
   if (!grads->w0 || !grads->b0 || !grads->w1) return FF_ERR_ARG;
   if (s->cap <= 0 || n_rays == 0) return FF_OK;
-  const unsigned blocks = rf_blocks(s->cap, 64, 16);
+  const unsigned blocks = rf_blocks(s->cap, 64, kBwdBlocksPerCu);
   if (!ws || ws_bytes < ((size_t)blocks + kReduceSplit) * kNerfPartial * 4) return FF_ERR_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   const ff_field_desc d = *desc;

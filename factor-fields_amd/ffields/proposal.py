@@ -5,6 +5,7 @@ All arithmetic runs in libffields.so (ff_proposal_sample / ff_proposal_bwd).
 """
 
 import ctypes
+import math
 
 import torch
 import torch.nn as nn
@@ -88,3 +89,66 @@ class ProposalNet(nn.Module):
                                         ctypes.byref(samples_struct), L.ptr(weights), float(loss_scale),
                                         ctypes.byref(g), L.ptr(loss_sum), L.ptr(ws), nb, L.stream_of(rays)),
                 "ff_proposal_bwd")
+
+
+class ProposalReferee:
+    """The occupancy sampler as fallback and referee of the learned proposal.
+
+    ``compare(model, rays, target)`` renders the first ``n_rays`` rays of a
+    training batch with the occupancy sampler and with the proposal (same
+    field, deterministic draws) and accumulates both squared errors on the
+    device.  The renders only read tables and weights, so they run on a side
+    stream (default priority) and overlap the batch's backward pass;
+    ``join()`` makes the current stream wait for them (call it before the
+    optimizer writes the tables).  ``decide()`` reads the accumulated errors
+    (one host synchronisation) and switches the proposal off for good when
+    PSNR(occupancy sampler) - PSNR(proposal) exceeds ``max_gap_db``.
+    """
+
+    def __init__(self, device, n_rays=256, max_gap_db=0.1, n_samples=-1):
+        self.n_rays = n_rays
+        self.max_gap_db = max_gap_db
+        self.n_samples = n_samples
+        self.sq_err = torch.zeros(2, device=device)      # occupancy sampler, proposal
+        self.stream = torch.cuda.Stream(device=device)
+        self._done = None
+        self.active = True
+        self.gaps_db = []
+
+    def compare(self, model, rays, target):
+        if not self.active or model.alphaMask is None:
+            return
+        main = torch.cuda.current_stream()
+        ready = torch.cuda.Event()
+        ready.record(main)
+        jr, jt = rays[:self.n_rays], target[:self.n_rays]
+        with torch.cuda.stream(self.stream):
+            self.stream.wait_event(ready)
+            occ = model.render_eval(jr, use_proposal=False, N_samples=self.n_samples)
+            prop = model.render_eval(jr, use_proposal=True)
+            self.sq_err[0] += ((occ - jt) ** 2).sum()
+            self.sq_err[1] += ((prop - jt) ** 2).sum()
+            self._done = torch.cuda.Event()
+            self._done.record(self.stream)
+        # the batch tensors are read on the side stream: keep their memory until it is done
+        jr.record_stream(self.stream)
+        jt.record_stream(self.stream)
+
+    def join(self):
+        if self._done is not None:
+            torch.cuda.current_stream().wait_event(self._done)
+            self._done = None
+
+    def decide(self):
+        """Returns the gap (dB) of the errors accumulated since the last call,
+        or None when there were none; deactivates the proposal above the bar."""
+        self.join()
+        s0, s1 = self.sq_err.tolist()
+        self.sq_err.zero_()
+        if not (s0 > 0 and s1 > 0):
+            return None
+        gap = 10 * math.log10(s1 / s0)      # PSNR(occupancy sampler) - PSNR(proposal)
+        self.gaps_db.append(round(gap, 4))
+        if gap > self.max_gap_db:
+            self.active = False
+        return gap

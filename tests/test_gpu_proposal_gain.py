@@ -62,3 +62,28 @@ def test_trained_proposal_beats_uniform_placement():
           f"occupancy sampler ({ns} max samples) {p_oc:.2f} dB", file=sys.stderr, flush=True)
     assert p_tr >= p_un + 1.0, (p_tr, p_un)
     assert p_tr >= p_oc - 0.5, (p_tr, p_oc)
+
+
+def test_referee_matches_same_stream_renders():
+    """ProposalReferee's side-stream renders give the same squared errors as
+    rendering on the current stream, and the fallback switch trips only above
+    the bar."""
+    from ffields.proposal import ProposalReferee
+    from test_gpu_render import make_model, make_rays
+    print("[case] proposal referee", file=sys.stderr, flush=True)
+    m = make_model(seed=3)
+    m.enable_proposal(reso=16, n_fine=48, table_dtype="fp32")
+    rays = make_rays(300, seed=40).cuda()
+    tgt = torch.rand((300, 3), generator=torch.Generator().manual_seed(41)).cuda()
+    ref = ProposalReferee(rays.device, n_rays=200, max_gap_db=0.1)
+    ref.compare(m, rays, tgt)
+    ref.join()
+    got = ref.sq_err.clone()
+    occ = m.render_eval(rays[:200], use_proposal=False)
+    prop = m.render_eval(rays[:200], use_proposal=True)
+    want = torch.stack([((occ - tgt[:200]) ** 2).sum(), ((prop - tgt[:200]) ** 2).sum()])
+    assert torch.allclose(got, want, rtol=1e-6, atol=0), (got, want)
+    gap = ref.decide()
+    assert abs(gap - 10 * math.log10(want[1].item() / want[0].item())) < 1e-4
+    assert ref.active == (gap <= 0.1)
+    assert ref.sq_err.abs().sum().item() == 0
