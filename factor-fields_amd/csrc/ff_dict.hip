@@ -1,0 +1,521 @@
+// Dictionary image-set field: image-set mode (models/FactorFields.py:286-288,
+// :469-470) with a shared 2-D basis of 144 channels (basis_dims
+// [32,32,32,16,16,16]), a per-image 3-D coefficient grid [n_images, h, w, 144]
+// (get_coeff, :425-465) and a 144 -> 64 -> O decoder (MLPMixer, :113-159), BF16
+// tables with FP32 accumulation.
+//
+// Decomposition: a wave handles 16 pixels with 4 lanes per pixel; lane
+// (pixel p, group g) owns the 4-channel pieces 16·mt + 4g .. +3 (mt = 0..8).
+// That ownership is the accumulator layout of v_mfma_f32_16x16x32_bf16
+// (lane l holds rows 4(l/16) .. +3 of column l%16), and, with the K order of
+// each 32-wide step permuted to slot (ks, g, j) -> 16(2ks + j/4) + 4g + j%4,
+// also its B-operand layout: the features, the hidden activations and the
+// gradients feed the next product straight from registers.
+//   forward:  H^T = W0a · X^T (W0a = [W0 | b0], X with a constant-1 column),
+//             out^T = W1 · relu(H^T)
+//   backward: dH^T = W1^T · dout^T (ReLU mask), dX^T = W0a^T · dH^T
+// Decoder weight gradients take K = pixels: X, relu(H), dH and dout are
+// transposed through LDS and accumulated by v_mfma_f32_32x32x16_bf16 in the
+// accumulation registers across the wave's tiles; each wave writes one partial
+// row, reduced afterwards in a fixed order.  Table gradients are staged in LDS
+// and scattered so that every atomic instruction carries four full 64-byte
+// lines (16 lanes on 16 consecutive channels of one texel).
+#include "ff_field_impl.h"
+
+namespace ff {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 dbf8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float df4;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float df16;
+
+constexpr int kDC = 144;  // channels
+constexpr int kDH = 64;   // hidden units
+constexpr int kDPx = 16;  // pixels per wave tile
+constexpr int kDMt = 9;   // 16-channel tiles of features (tile 9 carries the bias column)
+constexpr int kDWaves = 4;
+constexpr int kDOMax = 4;
+
+__device__ __forceinline__ int dlevel(int mt) { return mt < 2 ? 0 : (mt < 4 ? 1 : (mt < 6 ? 2 : mt - 3)); }
+__device__ __forceinline__ int doff(int L) { return L < 3 ? 32 * L : 96 + 16 * (L - 3); }
+__device__ __forceinline__ int ddim(int L) { return L < 3 ? 32 : 16; }
+
+__device__ __forceinline__ df4 mfma16(const dbf8& a, const dbf8& b, const df4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ df16 mfma32(const dbf8& a, const dbf8& b, const df16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// Orders this wave's LDS accesses across a hand-off between lanes: the wave's
+// LDS operations execute in order, so waiting for its outstanding LDS traffic
+// and keeping the compiler from moving memory accesses across is enough (the
+// waves of a block run different numbers of tiles, so no block barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// K slot (ks, g, j) of a 32-wide step -> index.
+__device__ __forceinline__ int dperm(int ks, int g, int j) { return 16 * (2 * ks + (j >> 2)) + 4 * g + (j & 3); }
+
+__device__ __forceinline__ float w0a(const ff_mlp_desc& m, int h, int k) {
+  return k < kDC ? m.w0[h * kDC + k] : (k == kDC ? m.b0[h] : 0.f);
+}
+
+// Block-shared A operands (BF16): layer 1 [hidden tile][K step][lane] and the
+// data gradient [channel tile][K step][lane].
+struct DictW {
+  dbf8 a1[4][5][64];
+  dbf8 a4[kDMt][2][64];
+};
+
+// Per-wave LDS area: weight-gradient operands (phase A, bf16 rows of 16 pixels
+// + 8 padding), then, aliased, the scatter staging (phase B).
+constexpr int kDRow = 24;
+struct DictPhaseA {
+  __bf16 xt[160][kDRow];
+  __bf16 gt[64][kDRow];
+  __bf16 ht[64][kDRow];
+  __bf16 dt[32][kDRow];
+};
+constexpr int kDVal = kDC + 1;  // staging row (odd stride: conflict-free column writes)
+struct DictPhaseB {
+  float db[kDPx][kDVal];
+  float dc[kDPx][kDVal];
+};
+union DictWave {
+  DictPhaseA a;
+  DictPhaseB b;
+};
+// Corners of the wave's pixels, written during the gather and read by the
+// scatter (not aliased with the areas above).
+struct DictCorners {
+  int btex[6][kDPx][4];
+  float bw[6][kDPx][4];
+  int ctex[kDPx][8];
+  float cw[kDPx][8];
+};
+
+__device__ __forceinline__ void load_dict_weights(const ff_mlp_desc& m, DictW& W) {
+  for (int e = threadIdx.x; e < 4 * 5 * 64 + kDMt * 2 * 64; e += blockDim.x) {
+    dbf8 v;
+    if (e < 4 * 5 * 64) {
+      const int l = e & 63, ks = (e >> 6) % 5, mt = e / 320;
+      const int r = l & 15, g = l >> 4;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = (__bf16)w0a(m, 16 * mt + r, dperm(ks, g, j));
+      W.a1[mt][ks][l] = v;
+    } else {
+      const int e2 = e - 4 * 5 * 64;
+      const int l = e2 & 63, ks = (e2 >> 6) & 1, mt = e2 >> 7;
+      const int r = l & 15, g = l >> 4;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = (__bf16)w0a(m, dperm(ks, g, j), 16 * mt + r);
+      W.a4[mt][ks][l] = v;
+    }
+  }
+}
+
+// Gathered values of one pixel's pieces (lane-owned channels).
+struct DictPix {
+  float bas[kDMt][4], cof[kDMt][4];
+};
+
+template <bool BF>
+__device__ __forceinline__ void load4(const void* t, int64_t e, float* v) { load_ch<4, BF>(t, e, v); }
+
+// Basis and coefficient values of the lane's pieces at pixel xv; the g = 0
+// lane of each pixel also records the corners in K (when K is given).
+template <bool BF>
+__device__ __forceinline__ void dict_gather(const ff_field_desc& fd, const float* xv, int g, int px, DictPix& P,
+                                            DictCorners* K) {
+#pragma unroll
+  for (int L = 0; L < 6; ++L) {
+    const int R = fd.basis_reso[L];
+    float pos[2];
+    const int Rs[2] = {R, R};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) pos[a] = unnorm_ac(map_coord(fd.mapping, xv[a], fd.map_origin[a], fd.map_scale[L]), R);
+    Corners<2> bc;
+    linear_corners<2>(pos, Rs, bc);
+    if (K && g == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        K->btex[L][px][k] = bc.tex[k];
+        K->bw[L][px][k] = bc.w[k];
+      }
+    }
+    const int dl = ddim(L);
+#pragma unroll
+    for (int mt = 0; mt < kDMt; ++mt) {
+      if (dlevel(mt) != L) continue;
+      const int base = 16 * mt - doff(L) + 4 * g;
+      float b[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float v[4];
+        load4<BF>(fd.basis[L], (int64_t)bc.tex[k] * dl + base, v);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i] = fmaf(v[i], bc.w[k], b[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) P.bas[mt][i] = b[i];
+    }
+  }
+  float pts[3], pos3[3];
+  coeff_pts<3>(fd, xv, pts);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) pos3[a] = unnorm_border(pts[a], fd.coeff_reso[a]);
+  Corners<3> cc;
+  linear_corners<3>(pos3, fd.coeff_reso, cc);
+  if (K && g == 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      K->ctex[px][k] = cc.tex[k];
+      K->cw[px][k] = cc.w[k];
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < kDMt; ++mt) {
+    float c[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (cc.w[k] != 0.f) {
+        float v[4];
+        load4<BF>(fd.coeff[0], (int64_t)cc.tex[k] * kDC + 16 * mt + 4 * g, v);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] = fmaf(v[i], cc.w[k], c[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) P.cof[mt][i] = c[i];
+  }
+}
+
+// Decoder forward for the wave's 16 pixels: returns the layer-1 accumulators
+// c1[hidden tile] and the output accumulator c2 (lane (p, g): outputs 4g..4g+3).
+__device__ __forceinline__ void dict_decoder_fwd(const DictW& W, const dbf8 (&a2)[2], const DictPix& P, int g,
+                                                 df4 (&c1)[4], df4& c2) {
+  const int lane = threadIdx.x & 63;
+  dbf8 xb[5];
+#pragma unroll
+  for (int ks = 0; ks < 5; ++ks)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int mt = 2 * ks + (j >> 2), i = j & 3;
+      float f;
+      if (mt < kDMt) f = P.bas[mt][i] * P.cof[mt][i];
+      else f = (g == 0 && i == 0) ? 1.f : 0.f;  // column 144: the bias
+      xb[ks][j] = (__bf16)f;
+    }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    df4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 5; ++ks) acc = mfma16(W.a1[mt][ks][lane], xb[ks], acc);
+    c1[mt] = acc;
+  }
+  df4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    dbf8 hb;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) hb[j] = (__bf16)fmaxf(c1[2 * ks + (j >> 2)][j & 3], 0.f);
+    acc = mfma16(a2[ks], hb, acc);
+  }
+  c2 = acc;
+}
+
+__device__ __forceinline__ void load_a2(const ff_mlp_desc& m, dbf8 (&a2)[2]) {
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a2[ks][j] = (__bf16)(r < m.out_dim ? m.w1[r * kDH + dperm(ks, g, j)] : 0.f);
+}
+
+template <bool BF>
+__global__ __launch_bounds__(256) void dict_fwd_kernel(ff_field_desc fd, ff_mlp_desc m,
+                                                       const float* __restrict__ x, int64_t n,
+                                                       float* __restrict__ out) {
+  __shared__ DictW W;
+  load_dict_weights(m, W);
+  __syncthreads();
+  dbf8 a2[2];
+  load_a2(m, a2);
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const int O = m.out_dim;
+  const int64_t ntiles = (n + kDPx - 1) / kDPx;
+  const int64_t wave = (int64_t)blockIdx.x * kDWaves + (threadIdx.x >> 6);
+  for (int64_t tile = wave; tile < ntiles; tile += (int64_t)gridDim.x * kDWaves) {
+    const int64_t p = tile * kDPx + r;
+    const bool active = p < n;
+    const int64_t pc = active ? p : n - 1;
+    const float xv[3] = {x[pc * 3 + 0], x[pc * 3 + 1], x[pc * 3 + 2]};
+    DictPix P;
+    dict_gather<BF>(fd, xv, g, r, P, nullptr);
+    df4 c1[4], c2;
+    dict_decoder_fwd(W, a2, P, g, c1, c2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int o = 4 * g + i;
+      if (active && o < O) out[p * O + o] = c2[i];
+    }
+  }
+}
+
+// Fused backward (mode 0: d_out given in dsrc) or regression training pass
+// (mode 1: MSE, mode 2: L1 against dsrc).
+template <bool BF>
+__global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_desc m, ff_grads gin,
+                                                       const float* __restrict__ x, int64_t n,
+                                                       const float* __restrict__ dsrc, int mode,
+                                                       float loss_scale, float* __restrict__ ws,
+                                                       float* __restrict__ loss_sum) {
+  __shared__ DictW W;
+  __shared__ __attribute__((aligned(16))) DictWave WV[kDWaves];
+  __shared__ DictCorners WK[kDWaves];
+  load_dict_weights(m, W);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4, l32 = lane & 31, h32 = lane >> 5;
+  const int wib = threadIdx.x >> 6;
+  DictWave& S = WV[wib];
+  DictCorners& K = WK[wib];
+  const int O = m.out_dim;
+  dbf8 a2[2], a3[4];
+  load_a2(m, a2);
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int o = 8 * g + j;
+      a3[mt][j] = (__bf16)(o < O ? m.w1[o * kDH + 16 * mt + r] : 0.f);
+    }
+  df16 acc0[5][2], acc1[2];
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc0[a][b][i] = 0.f;
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc1[b][i] = 0.f;
+  const ff_grads g2 = local_grads(gin);
+  const float grad_scale = (float)(1.0 / ((double)n * O)) * loss_scale;
+  float lsum = 0.f;
+  const int64_t ntiles = (n + kDPx - 1) / kDPx;
+  const int64_t wave = (int64_t)blockIdx.x * kDWaves + wib;
+  const int64_t nwaves = (int64_t)gridDim.x * kDWaves;
+  for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
+    const int64_t p = tile * kDPx + r;
+    const bool active = p < n;
+    const int64_t pc = active ? p : n - 1;
+    const float xv[3] = {x[pc * 3 + 0], x[pc * 3 + 1], x[pc * 3 + 2]};
+    DictPix P;
+    dict_gather<BF>(fd, xv, g, r, P, &K);
+    df4 c1[4], c2;
+    dict_decoder_fwd(W, a2, P, g, c1, c2);
+    // output gradient (lane (p, g) holds outputs 4g .. 4g+3)
+    float d[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int o = 4 * g + i;
+      float v = 0.f;
+      if (active && o < O) {
+        if (mode == 0) {
+          v = dsrc[pc * O + o];
+        } else {
+          const float e = c2[i] - dsrc[pc * O + o];
+          if (mode == 1) {
+            v = 2.f * e * grad_scale;
+            lsum += e * e;
+          } else {
+            v = (e > 0.f ? 1.f : (e < 0.f ? -1.f : 0.f)) * grad_scale;
+            lsum += fabsf(e);
+          }
+        }
+      }
+      d[i] = v;
+    }
+    // dH^T = W1^T . dout^T (outputs 0..3 sit in the g = 0 lanes), ReLU mask
+    dbf8 bd;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) bd[j] = (__bf16)((g == 0 && j < 4) ? d[j] : 0.f);
+    df4 dh[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      df4 z = {0.f, 0.f, 0.f, 0.f};
+      df4 t = mfma16(a3[mt], bd, z);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) t[i] = c1[mt][i] > 0.f ? t[i] : 0.f;
+      dh[mt] = t;
+    }
+    dbf8 bdh[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bdh[ks][j] = (__bf16)dh[2 * ks + (j >> 2)][j & 3];
+    // weight-gradient operands through LDS (phase A)
+#pragma unroll
+    for (int mt = 0; mt < 10; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float f;
+        if (mt < kDMt) f = P.bas[mt][i] * P.cof[mt][i];
+        else f = (g == 0 && i == 0) ? 1.f : 0.f;
+        S.a.xt[16 * mt + 4 * g + i][r] = (__bf16)(active ? f : 0.f);
+      }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        S.a.ht[16 * mt + 4 * g + i][r] = (__bf16)fmaxf(c1[mt][i], 0.f);
+        S.a.gt[16 * mt + 4 * g + i][r] = (__bf16)dh[mt][i];
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      S.a.dt[4 * g + i][r] = (__bf16)(g == 0 ? d[i] : 0.f);
+      S.a.dt[16 + 4 * g + i][r] = (__bf16)0.f;
+    }
+    wave_lds_sync();
+    {
+      const dbf8 bdt = *reinterpret_cast<const dbf8*>(&S.a.dt[l32][8 * h32]);
+#pragma unroll
+      for (int nh = 0; nh < 2; ++nh) {
+        const dbf8 bg = *reinterpret_cast<const dbf8*>(&S.a.gt[32 * nh + l32][8 * h32]);
+#pragma unroll
+        for (int mk = 0; mk < 5; ++mk) {
+          const dbf8 ax = *reinterpret_cast<const dbf8*>(&S.a.xt[32 * mk + l32][8 * h32]);
+          acc0[mk][nh] = mfma32(ax, bg, acc0[mk][nh]);
+        }
+        const dbf8 ah = *reinterpret_cast<const dbf8*>(&S.a.ht[32 * nh + l32][8 * h32]);
+        acc1[nh] = mfma32(ah, bdt, acc1[nh]);
+      }
+    }
+    // data gradient dX^T = W0a^T . dH^T: lane (p, g) receives its own pieces
+    float df[kDMt][4];
+#pragma unroll
+    for (int mt = 0; mt < kDMt; ++mt) {
+      df4 t = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) t = mfma16(W.a4[mt][ks][lane], bdh[ks], t);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) df[mt][i] = t[i];
+    }
+    wave_lds_sync();
+    // phase B: stage d(basis), d(coeff) and the corners, then line-wide atomics
+#pragma unroll
+    for (int mt = 0; mt < kDMt; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c = 16 * mt + 4 * g + i;
+        S.b.db[r][c] = active ? df[mt][i] * P.cof[mt][i] : 0.f;
+        S.b.dc[r][c] = active ? df[mt][i] * P.bas[mt][i] : 0.f;
+      }
+    wave_lds_sync();
+#pragma unroll
+    for (int L = 0; L < 6; ++L) {
+      const int dl = ddim(L), nb = dl / 16, per = 4 * nb;
+      float* dst = g2.basis[L];
+      for (int it = g; it < kDPx * per; it += 4) {
+        const int px = it / per, rem = it - px * per, k = rem / nb, blk = rem - k * nb;
+        const float v = K.bw[L][px][k] * S.b.db[px][doff(L) + 16 * blk + r];
+        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)K.btex[L][px][k] * dl + 16 * blk + r, v);
+      }
+    }
+    {
+      float* dst = g2.coeff[0];
+      for (int it = g; it < kDPx * 8 * kDMt; it += 4) {
+        const int px = it / (8 * kDMt), rem = it - px * 8 * kDMt, k = rem / kDMt, blk = rem - k * kDMt;
+        const float w = K.cw[px][k];
+        if (w == 0.f) continue;
+        const float v = w * S.b.dc[px][16 * blk + r];
+        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)K.ctex[px][k] * kDC + 16 * blk + r, v);
+      }
+    }
+    wave_lds_sync();
+  }
+  // partial row: dW0 [64][144] | db0 [64] | dW1 [O][64]
+  float* row = ws + wave * (int64_t)(kDH * kDC + kDH + O * kDH);
+#pragma unroll
+  for (int mk = 0; mk < 5; ++mk)
+#pragma unroll
+    for (int nh = 0; nh < 2; ++nh)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int k = 32 * mk + 8 * (i >> 2) + 4 * h32 + (i & 3), h = 32 * nh + l32;
+        if (k < kDC) row[h * kDC + k] = acc0[mk][nh][i];
+        else if (k == kDC) row[kDH * kDC + h] = acc0[mk][nh][i];
+      }
+#pragma unroll
+  for (int mh = 0; mh < 2; ++mh)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int h = 32 * mh + 8 * (i >> 2) + 4 * h32 + (i & 3), o = l32;
+      if (o < O) row[kDH * kDC + kDH + o * kDH + h] = acc1[mh][i];
+    }
+  if (mode != 0) {
+    lsum = wave_sum(lsum);
+    if (lane == 0 && loss_sum) unsafeAtomicAdd(loss_sum, lsum);
+  }
+}
+
+int dict_cu() {
+  int dev = 0, cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0) return 256;
+  return cu;
+}
+
+// One block of four waves per CU is resident (LDS, registers).
+unsigned dict_blocks(int64_t n) {
+  const int64_t tiles = (n + kDPx - 1) / kDPx;
+  const int64_t need = (tiles + kDWaves - 1) / kDWaves;
+  const int64_t cap = dict_cu();
+  return (unsigned)(need < cap ? (need > 0 ? need : 1) : cap);
+}
+
+int dict_partial(int O) { return kDH * kDC + kDH + O * kDH; }
+
+}  // namespace
+
+bool dict_supported(const ff_field_desc& d, const ff_mlp_desc& m) {
+  static const int dims[6] = {32, 32, 32, 16, 16, 16};
+  if (d.table_dtype != FF_BF16) return false;  // the decoder runs in BF16: only for BF16 tables
+  if (d.in_dim != 2 || d.x_dim != 3 || d.n_levels != 6 || d.n_channels != kDC) return false;
+  if (d.basis_kind != FF_BASIS_GRID || d.basis_interp != FF_INTERP_BILINEAR) return false;
+  if (d.coeff_kind != FF_COEFF_GRID || d.coeff_ndim != 3 || d.coeff_interp != FF_INTERP_BILINEAR) return false;
+  for (int l = 0; l < 6; ++l)
+    if (d.basis_dims[l] != dims[l] || !d.basis[l] || d.basis_reso[l] < 2) return false;
+  if (!d.coeff[0]) return false;
+  return m.n_layers == 2 && m.in_dim == kDC && m.hidden == kDH && m.out_dim >= 1 && m.out_dim <= kDOMax &&
+         m.w0 && m.b0 && m.w1;
+}
+
+size_t dict_workspace_bytes(const ff_mlp_desc& m, int64_t n) {
+  if (n <= 0) return 0;
+  return ((size_t)dict_blocks(n) * kDWaves + kReduceSplit) * dict_partial(m.out_dim) * sizeof(float);
+}
+
+int launch_dict_fwd(const ff_field_desc& d, const ff_mlp_desc& m, const float* x, int64_t n, float* out,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(dict_fwd_kernel<true>, dim3(dict_blocks(n)), dim3(64 * kDWaves), 0, s, d, m, x, n, out);
+  return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
+}
+
+int launch_dict_bwd(const ff_field_desc& d, const ff_mlp_desc& m, const ff_grads& g, const float* x, int64_t n,
+                    const float* dsrc, int mode, float loss_scale, void* ws, size_t ws_bytes, float* loss_sum,
+                    hipStream_t s) {
+  if (!ws || ws_bytes < dict_workspace_bytes(m, n)) return FF_ERR_WORKSPACE;
+  const unsigned blocks = dict_blocks(n);
+  hipLaunchKernelGGL(dict_bwd_kernel<true>, dim3(blocks), dim3(64 * kDWaves), 0, s, d, m, g, x, n, dsrc, mode,
+                     loss_scale, (float*)ws, loss_sum);
+  if (hipGetLastError() != hipSuccess) return FF_ERR_LAUNCH;
+  return launch_reduce_partials((const float*)ws, (int)(blocks * kDWaves), 2, kDC, kDH, m.out_dim, g, s);
+}
+
+}  // namespace ff

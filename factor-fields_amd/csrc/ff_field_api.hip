@@ -210,6 +210,8 @@ int field_bwd_common(const ff_field_desc* desc, const ff_mlp_desc* mlp, const fl
   hipStream_t s = (hipStream_t)stream;
   const ff_field_desc& d = *desc;
   const ff_mlp_desc& m = *mlp;
+  if (dict_supported(d, m))
+    return launch_dict_bwd(d, m, *grads, x, n, dsrc, mode, loss_scale, ws, ws_bytes, loss_sum, s);
   return with_field(d, [&](auto Di, auto CKi, auto BFi) {
     constexpr int D = decltype(Di)::value, CK = decltype(CKi)::value;
     constexpr bool BF = decltype(BFi)::value;
@@ -349,6 +351,7 @@ int ff_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const float*
   hipStream_t s = (hipStream_t)stream;
   const ff_field_desc& d = *desc;
   const ff_mlp_desc& m = *mlp;
+  if (dict_supported(d, m)) return launch_dict_fwd(d, m, x, n, out, s);
   return with_field(d, [&](auto Di, auto CKi, auto BFi) {
     constexpr int D = decltype(Di)::value, CK = decltype(CKi)::value;
     constexpr bool BF = decltype(BFi)::value;
@@ -361,6 +364,7 @@ int ff_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const float*
 
 int ff_field_fused_supported(const ff_field_desc* desc, const ff_mlp_desc* mlp) {
   if (check_desc(desc) || check_mlp(mlp)) return 0;
+  if (dict_supported(*desc, *mlp)) return 1;
   const int st = with_field(*desc, [&](auto, auto, auto) {
     return with_mlp(*mlp, desc->n_channels, [&](auto, auto, auto) { return (int)FF_OK; });
   });
@@ -369,6 +373,7 @@ int ff_field_fused_supported(const ff_field_desc* desc, const ff_mlp_desc* mlp) 
 
 size_t ff_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp, int64_t n) {
   if (check_desc(desc) || check_mlp(mlp) || n <= 0) return 0;
+  if (dict_supported(*desc, *mlp)) return dict_workspace_bytes(*mlp, n);
   return ff_linear_mat_bwd_workspace_bytes(mlp, n);
 }
 

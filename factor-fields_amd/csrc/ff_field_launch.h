@@ -35,6 +35,16 @@ int launch_mlp_bwd(const ff_mlp_desc& m, const float* f, int64_t n, const float*
 
 // Deterministic reduction of per-wave decoder weight-gradient partials into
 // the gradient buffers (ff_field_api.hip).
+// Dictionary image-set fast path (ff_dict.hip): image-set mode, basis_dims
+// [32,32,32,16,16,16], BF16 tables, decoder 144 -> 64 -> O (O <= 4).
+bool dict_supported(const ff_field_desc& d, const ff_mlp_desc& m);
+size_t dict_workspace_bytes(const ff_mlp_desc& m, int64_t n);
+int launch_dict_fwd(const ff_field_desc& d, const ff_mlp_desc& m, const float* x, int64_t n, float* out,
+                    hipStream_t s);
+int launch_dict_bwd(const ff_field_desc& d, const ff_mlp_desc& m, const ff_grads& g, const float* x, int64_t n,
+                    const float* dsrc, int mode, float loss_scale, void* ws, size_t ws_bytes, float* loss_sum,
+                    hipStream_t s);
+
 // Deterministic two-level sum of per-wave partial rows [rows][P] (P = decoder
 // parameters of an NL-layer IN -> H -> O MLP) into g.w0 / g.b0 / g.w1; the
 // workspace holds the rows followed by kReduceSplit rows of scratch.

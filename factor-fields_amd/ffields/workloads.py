@@ -63,32 +63,54 @@ def nerf_cfg():
 
 
 # ---------------------------------------------------------------- 2D image
-def synth_image(size=512, seed=BASE_SEED, n_gauss=64, n_sin=32, device="cpu"):
+def synth_image(size=512, seed=BASE_SEED, n_gauss=64, n_sin=32, device="cpu", compute_device="cpu"):
     """[size, size, 3] float32 in [0,1]: 64 random 2D Gaussians (sigma U[4,64]
     px) + 32 random sinusoids (U[1,40] cycles/image), min-max normalised per
-    channel (scaled to `size` from the 512 px definition)."""
+    channel (scaled to `size` from the 512 px definition).  The random draws
+    come from a CPU generator; the evaluation (float64) runs on compute_device."""
     g = torch.Generator().manual_seed(seed)
     s = size / 512.0
-    yy, xx = torch.meshgrid(torch.arange(size, dtype=torch.float64) + 0.5,
-                            torch.arange(size, dtype=torch.float64) + 0.5, indexing="ij")
-    img = torch.zeros((size, size, 3), dtype=torch.float64)
+    cd = torch.device(compute_device)
+    yy, xx = torch.meshgrid(torch.arange(size, dtype=torch.float64, device=cd) + 0.5,
+                            torch.arange(size, dtype=torch.float64, device=cd) + 0.5, indexing="ij")
+    img = torch.zeros((size, size, 3), dtype=torch.float64, device=cd)
     for _ in range(n_gauss):
-        c = torch.rand(2, generator=g, dtype=torch.float64) * size
-        sig = (4 + torch.rand(1, generator=g, dtype=torch.float64) * 60) * s
-        amp = torch.rand(3, generator=g, dtype=torch.float64)
+        c = (torch.rand(2, generator=g, dtype=torch.float64) * size).to(cd)
+        sig = ((4 + torch.rand(1, generator=g, dtype=torch.float64) * 60) * s).to(cd)
+        amp = torch.rand(3, generator=g, dtype=torch.float64).to(cd)
         r2 = (xx - c[0]) ** 2 + (yy - c[1]) ** 2
         img += torch.exp(-r2 / (2 * sig ** 2))[..., None] * amp
     for _ in range(n_sin):
-        f = 1 + torch.rand(1, generator=g, dtype=torch.float64) * 39
-        th = torch.rand(1, generator=g, dtype=torch.float64) * 2 * math.pi
-        ph = torch.rand(1, generator=g, dtype=torch.float64) * 2 * math.pi
-        amp = torch.rand(3, generator=g, dtype=torch.float64) / math.sqrt(n_sin)
+        f = (1 + torch.rand(1, generator=g, dtype=torch.float64) * 39).to(cd)
+        th = (torch.rand(1, generator=g, dtype=torch.float64) * 2 * math.pi).to(cd)
+        ph = (torch.rand(1, generator=g, dtype=torch.float64) * 2 * math.pi).to(cd)
+        amp = (torch.rand(3, generator=g, dtype=torch.float64) / math.sqrt(n_sin)).to(cd)
         u = (xx * torch.cos(th) + yy * torch.sin(th)) / size
         img += torch.sin(2 * math.pi * f * u + ph)[..., None] * amp
     lo = img.amin(dim=(0, 1), keepdim=True)
     hi = img.amax(dim=(0, 1), keepdim=True)
     img = (img - lo) / (hi - lo)
     return img.float().to(device)
+
+
+def synth_image_set(n=1024, size=256, device="cuda"):
+    """The dictionary image set: n images [size, size, 3] from synth_image at
+    size/512 scale, image i seeded BASE_SEED + 4000 + i.  Returns [n, size*size, 3]
+    (row-major pixels)."""
+    out = torch.empty((n, size * size, 3), device=device)
+    for i in range(n):
+        out[i] = synth_image(size, seed=BASE_SEED + 4000 + i, device=device,
+                             compute_device=device).reshape(-1, 3)
+    return out
+
+
+def dict_cfg(n_images=1024, size=256):
+    """Dictionary image set: shared 2-D basis [32,32,32,16,16,16]@[32..128]
+    (144 channels, sawtooth), per-image 32x32x144 coefficient grids, decoder
+    144 -> 64 -> 3, 2^22 pixels per step."""
+    return _cfg("images", dict(basis_dims=[32, 32, 32, 16, 16, 16], coeff_reso=32, out_dim=3, coef_init=1.0),
+                dict(aabb=[[0.0, 0.0, 0.0], [float(size), float(size), float(n_images)]]),
+                training=dict(batch_size=1 << 22, n_iters=2000, lr_small=0.02, lr_large=0.02))
 
 
 def pixel_coords(h, w, device="cpu"):
