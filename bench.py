@@ -353,8 +353,12 @@ def run_dict(args):
             opt.step(zero_grad=True)
 
         log(f"dict {form}: {args.dict_steps} training steps ({'fused' if fused else 'generic'} kernels)")
-        for _ in range(args.dict_steps):
+        t_tr = time.perf_counter()
+        for it in range(args.dict_steps):
             step()
+            if it % 20 == 19:
+                torch.cuda.synchronize()
+                log(f"dict {form}: step {it + 1}, {(time.perf_counter() - t_tr) / (it + 1) * 1e3:.1f} ms/step")
         torch.cuda.synchronize()
         timed = args.steps if form == "bf16" else 0
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(timed)]
@@ -385,8 +389,8 @@ def run_dict(args):
     out = {"metric": METRIC, "value": b["pts_per_s"], "unit": "field pts/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.dict_steps, "ms_per_step": b["ms_per_step"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32",
-           "data": "synthetic (1,024 seeded Gaussian+sinusoid images, 256x256)",
-           "config": {"workload": "dictionary_image_set_bf16: 1,024 images, basis [32,32,32,16,16,16]@[32..128], "
+           "data": f"synthetic ({N} seeded Gaussian+sinusoid images, 256x256)",
+           "config": {"workload": f"dictionary_image_set_bf16: {N} images, basis [32,32,32,16,16,16]@[32..128], "
                                   "32x32x144 coefficient per image, decoder 144-64-3, 2^22 px/step",
                       "forms": res},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
