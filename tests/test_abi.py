@@ -116,3 +116,26 @@ def test_proposal_bwd_workspace_checked():
                              ctypes.c_void_p(0x5000), 1.0, ctypes.byref(g), None, ctypes.c_void_p(0x6000), 1 << 30,
                              None)
     assert st == L.FF_ERR_ARG
+
+
+def test_sampler_limits_rejected():
+    """The sampler holds at most 2,048 samples per ray (32 rounds of 64) and
+    the proposal at most 64 fine samples: larger requests fail with
+    FF_ERR_ARG before any work is queued."""
+    lib = L.lib()
+    s = L.RaySampler()
+    for k in range(3):
+        s.aabb_min[k], s.aabb_max[k] = -1.0, 1.0
+    s.step_size = 0.01
+    s.distance_scale = 25.0
+    out = L.Samples()
+    out.xyz = out.z = out.dist = out.ray = out.idx = out.ray_off = 0x1000
+    out.cap = 16 * 4096
+    for ns, want in ((2049, L.FF_ERR_ARG), (0, L.FF_ERR_ARG)):
+        s.n_samples = ns
+        st = lib.ff_sample_rays(ctypes.byref(s), ctypes.c_void_p(0x2000), None, 16, ctypes.byref(out), None)
+        assert st == want, (ns, st)
+    s.n_samples = 2048
+    out.cap = 16 * 2048 - 1                       # capacity one short of n_rays * n_samples
+    st = lib.ff_sample_rays(ctypes.byref(s), ctypes.c_void_p(0x2000), None, 16, ctypes.byref(out), None)
+    assert st == L.FF_ERR_ARG
