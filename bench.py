@@ -307,9 +307,11 @@ def run_nerf(args):
 def run_dict(args):
     """Dictionary image set: 1,024 seeded 256x256 images sharing one 144-channel
     2-D basis with per-image 32x32x144 coefficient grids, decoder 144-64-3,
-    2^22 pixels per step in one launch, MSE, Adam.  The BF16 form (fused
-    matrix-core kernel) is timed; both forms are trained for the same steps and
-    their PSNR on 16 whole images is reported."""
+    2^22 pixels per step in one launch, MSE, Adam.  Pixels and images are
+    drawn uniformly (dataLoader/image_set.py:55-56); each batch is then ordered
+    by pixel position (the sort is inside the timed step).  The BF16 form
+    (fused matrix-core kernel) is timed; both forms are trained for the same
+    steps and their PSNR on 16 whole images is reported."""
     import torch
     from ffields import FactorFields, FusedAdam
     from ffields import workloads as W
@@ -328,6 +330,11 @@ def run_dict(args):
     def batch(gen):
         img = torch.randint(0, N, (n,), device=dev, generator=gen)
         p = torch.randint(0, S * S, (n,), device=dev, generator=gen)
+        if args.dict_sort:
+            # same samples, ordered by pixel position: pixels that share basis
+            # texels land in the same wave, whose scatter merges their atomics
+            p, order = torch.sort(p)
+            img = img[order]
         x = torch.stack([(p % S).float() + 0.5, (p // S).float() + 0.5, img.float() + 0.5], -1)
         return x, imgs[img, p]
 
@@ -553,6 +560,7 @@ def main():
     ap.add_argument("--dict-batch", type=int, default=1 << 22)
     ap.add_argument("--dict-steps", type=int, default=100, help="training steps before the evaluation")
     ap.add_argument("--dict-fp32", type=int, default=1, help="also train the FP32 form (generic kernels)")
+    ap.add_argument("--dict-sort", type=int, default=1, help="order each batch by pixel position")
     ap.add_argument("--nerf-form", default="both", choices=["both", "plain", "proposal"])
     ap.add_argument("--nerf-pool", type=int, default=1 << 20, help="ground-truth rays the batches are drawn from")
     ap.add_argument("--image-iters", type=int, default=2000)

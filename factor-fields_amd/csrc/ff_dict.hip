@@ -417,24 +417,35 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
         S.b.dc[r][c] = active ? df[mt][i] * P.bas[mt][i] : 0.f;
       }
     wave_lds_sync();
+    // Consecutive pixels whose corner k is the same texel (a batch ordered by
+    // pixel position shares basis texels) form a run: only the run's first
+    // pixel issues the atomic, with the run's sum.
 #pragma unroll
     for (int L = 0; L < 6; ++L) {
       const int dl = ddim(L), nb = dl / 16, per = 4 * nb;
       float* dst = g2.basis[L];
       for (int it = g; it < kDPx * per; it += 4) {
         const int px = it / per, rem = it - px * per, k = rem / nb, blk = rem - k * nb;
-        const float v = K.bw[L][px][k] * S.b.db[px][doff(L) + 16 * blk + r];
-        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)K.btex[L][px][k] * dl + 16 * blk + r, v);
+        const int t = K.btex[L][px][k];
+        if (px > 0 && K.btex[L][px - 1][k] == t) continue;
+        const int c = doff(L) + 16 * blk + r;
+        float v = 0.f;
+        for (int q = px; q < kDPx && K.btex[L][q][k] == t; ++q) v = fmaf(K.bw[L][q][k], S.b.db[q][c], v);
+        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)t * dl + 16 * blk + r, v);
       }
     }
     {
       float* dst = g2.coeff[0];
       for (int it = g; it < kDPx * 8 * kDMt; it += 4) {
         const int px = it / (8 * kDMt), rem = it - px * 8 * kDMt, k = rem / kDMt, blk = rem - k * kDMt;
-        const float w = K.cw[px][k];
-        if (w == 0.f) continue;
-        const float v = w * S.b.dc[px][16 * blk + r];
-        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)K.ctex[px][k] * kDC + 16 * blk + r, v);
+        if (K.cw[px][k] == 0.f) continue;
+        const int t = K.ctex[px][k];
+        if (px > 0 && K.ctex[px - 1][k] == t && K.cw[px - 1][k] != 0.f) continue;
+        const int c = 16 * blk + r;
+        float v = 0.f;
+        for (int q = px; q < kDPx && K.ctex[q][k] == t && K.cw[q][k] != 0.f; ++q)
+          v = fmaf(K.cw[q][k], S.b.dc[q][c], v);
+        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)t * kDC + 16 * blk + r, v);
       }
     }
     wave_lds_sync();

@@ -93,14 +93,58 @@ def synth_image(size=512, seed=BASE_SEED, n_gauss=64, n_sin=32, device="cpu", co
     return img.float().to(device)
 
 
-def synth_image_set(n=1024, size=256, device="cuda"):
-    """The dictionary image set: n images [size, size, 3] from synth_image at
-    size/512 scale, image i seeded BASE_SEED + 4000 + i.  Returns [n, size*size, 3]
-    (row-major pixels)."""
-    out = torch.empty((n, size * size, 3), device=device)
-    for i in range(n):
-        out[i] = synth_image(size, seed=BASE_SEED + 4000 + i, device=device,
-                             compute_device=device).reshape(-1, 3)
+def _image_params(seed, n_gauss=64, n_sin=32):
+    """synth_image's random draws for one image, in its order (CPU generator)."""
+    g = torch.Generator().manual_seed(seed)
+    gs, ss = [], []
+    for _ in range(n_gauss):
+        c = torch.rand(2, generator=g, dtype=torch.float64)
+        sig = torch.rand(1, generator=g, dtype=torch.float64)
+        amp = torch.rand(3, generator=g, dtype=torch.float64)
+        gs.append(torch.cat([c, sig, amp]))
+    for _ in range(n_sin):
+        f = torch.rand(1, generator=g, dtype=torch.float64)
+        th = torch.rand(1, generator=g, dtype=torch.float64)
+        ph = torch.rand(1, generator=g, dtype=torch.float64)
+        amp = torch.rand(3, generator=g, dtype=torch.float64)
+        ss.append(torch.cat([f, th, ph, amp]))
+    return torch.stack(gs), torch.stack(ss)
+
+
+def synth_image_set(n=1024, size=256, device="cuda", chunk=8):
+    """The dictionary image set: n images [size, size, 3], image i equal to
+    synth_image(size, seed=BASE_SEED + 4000 + i) (the same draws, evaluated in
+    float64 on `device`, `chunk` images at a time).  Returns
+    [n, size*size, 3] float32 (row-major pixels)."""
+    dev = torch.device(device)
+    sc = size / 512.0
+    params = [_image_params(BASE_SEED + 4000 + i) for i in range(n)]
+    G = torch.stack([p[0] for p in params]).to(dev)        # [n, 64, 6]: cx, cy, sig, amp rgb
+    Sn = torch.stack([p[1] for p in params]).to(dev)       # [n, 32, 6]: f, th, ph, amp rgb
+    yy, xx = torch.meshgrid(torch.arange(size, dtype=torch.float64, device=dev) + 0.5,
+                            torch.arange(size, dtype=torch.float64, device=dev) + 0.5, indexing="ij")
+    xx, yy = xx.reshape(-1), yy.reshape(-1)
+    out = torch.empty((n, size * size, 3), device=dev)
+    for i0 in range(0, n, chunk):
+        g = G[i0:i0 + chunk]
+        sn = Sn[i0:i0 + chunk]
+        k = g.shape[0]
+        img = torch.zeros((k, size * size, 3), dtype=torch.float64, device=dev)
+        for j in range(g.shape[1]):       # same accumulation order as synth_image
+            cx, cy = g[:, j, 0:1] * size, g[:, j, 1:2] * size
+            sig = (4 + g[:, j, 2:3] * 60) * sc
+            r2 = (xx[None] - cx) ** 2 + (yy[None] - cy) ** 2
+            img += torch.exp(-r2 / (2 * sig ** 2))[..., None] * g[:, j, None, 3:6]
+        for j in range(sn.shape[1]):
+            f = 1 + sn[:, j, 0:1] * 39
+            th = sn[:, j, 1:2] * 2 * math.pi
+            ph = sn[:, j, 2:3] * 2 * math.pi
+            amp = sn[:, j, 3:6] / math.sqrt(sn.shape[1])
+            u = (xx[None] * torch.cos(th) + yy[None] * torch.sin(th)) / size
+            img += torch.sin(2 * math.pi * f * u + ph)[..., None] * amp[:, None, :]
+        lo = img.amin(dim=1, keepdim=True)
+        hi = img.amax(dim=1, keepdim=True)
+        out[i0:i0 + k] = ((img - lo) / (hi - lo)).float()
     return out
 
 

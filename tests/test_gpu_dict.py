@@ -67,18 +67,29 @@ def test_dict_forward():
     assert rel.median().item() < 1e-4, rel.median().item()
 
 
-@pytest.mark.parametrize("how", ["regression_mse", "autograd"])
+def repeated_points(n, seed, W=8, H=6, D=5):
+    """Pixels drawn from a few pixel centres across all images, ordered by
+    position: consecutive pixels share basis texels (and some coefficient
+    texels), so the scatter's run merging is exercised."""
+    g = torch.Generator().manual_seed(seed)
+    pos = torch.tensor([[0.5, 0.5], [3.5, 2.5], [7.5, 5.5], [4.5, 1.5]])
+    k = torch.sort(torch.randint(0, len(pos), (n,), generator=g)).values
+    img = torch.randint(0, D, (n, 1), generator=g).float() + 0.5
+    return torch.cat([pos[k], img], 1)
+
+
+@pytest.mark.parametrize("how", ["regression_mse", "autograd", "regression_runs"])
 def test_dict_backward(how):
     m = make()
     n = 2503
-    x = image_points(n, seed=5)
+    x = repeated_points(n, seed=5) if how == "regression_runs" else image_points(n, seed=5)
     tgt = torch.rand((n, 3), generator=torch.Generator().manual_seed(6))
     r = torch.randn((n, 3), generator=torch.Generator().manual_seed(7))
     m.zero_grad(set_to_none=True)
     of = oracle_of(m, bf16_tables=True).requires_grad_(True)
     feats = of.get_coding(x)[0]
     out, pre, hb, xb, W0a, W1 = emulate(m, feats.detach())
-    if how == "regression_mse":
+    if how.startswith("regression"):
         m.regression_step(x.cuda(), tgt.cuda(), loss="mse")
         dout = 2 * (out - tgt) / (n * 3)
     else:
