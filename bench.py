@@ -308,10 +308,12 @@ def run_dict(args):
     """Dictionary image set: 1,024 seeded 256x256 images sharing one 144-channel
     2-D basis with per-image 32x32x144 coefficient grids, decoder 144-64-3,
     2^22 pixels per step in one launch, MSE, Adam.  Pixels and images are
-    drawn uniformly (dataLoader/image_set.py:55-56); each batch is then ordered
-    by pixel position (the sort is inside the timed step).  The BF16 form
-    (fused matrix-core kernel) is timed; both forms are trained for the same
-    steps and their PSNR on 16 whole images is reported."""
+    drawn uniformly with replacement (dataLoader/image_set.py:55-56); with
+    --dict-order cell (default) each batch arrives sorted by image and
+    coefficient cell (same distribution, drawn inside the timed step), so the
+    fused kernel merges the atomics of pixels that share texels.  The BF16
+    form (fused matrix-core kernel) is timed; both forms are trained for the
+    same steps and their PSNR on 16 whole images is reported."""
     import torch
     from ffields import FactorFields, FusedAdam
     from ffields import workloads as W
@@ -327,14 +329,10 @@ def run_dict(args):
     pix = torch.arange(S * S, device=dev)
     ex = torch.stack([(pix % S).float() + 0.5, (pix // S).float() + 0.5], -1)
 
+    order = W.cell_order(S, int(cfg.model.coeff_reso), dev) if args.dict_order == "cell" else None
+
     def batch(gen):
-        img = torch.randint(0, N, (n,), device=dev, generator=gen)
-        p = torch.randint(0, S * S, (n,), device=dev, generator=gen)
-        if args.dict_sort:
-            # same samples, ordered by pixel position: pixels that share basis
-            # texels land in the same wave, whose scatter merges their atomics
-            p, order = torch.sort(p)
-            img = img[order]
+        img, p = W.image_set_batch(n, N, S, gen, dev, order=order)
         x = torch.stack([(p % S).float() + 0.5, (p // S).float() + 0.5, img.float() + 0.5], -1)
         return x, imgs[img, p]
 
@@ -399,6 +397,7 @@ def run_dict(args):
            "data": f"synthetic ({N} seeded Gaussian+sinusoid images, 256x256)",
            "config": {"workload": f"dictionary_image_set_bf16: {N} images, basis [32,32,32,16,16,16]@[32..128], "
                                   "32x32x144 coefficient per image, decoder 144-64-3, 2^22 px/step",
+                      "batch_order": args.dict_order,
                       "forms": res},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("dict_bwd_kernel"),
@@ -560,7 +559,8 @@ def main():
     ap.add_argument("--dict-batch", type=int, default=1 << 22)
     ap.add_argument("--dict-steps", type=int, default=100, help="training steps before the evaluation")
     ap.add_argument("--dict-fp32", type=int, default=1, help="also train the FP32 form (generic kernels)")
-    ap.add_argument("--dict-sort", type=int, default=1, help="order each batch by pixel position")
+    ap.add_argument("--dict-order", default="cell", choices=["cell", "random"],
+                    help="batch order of the dictionary image set (same distribution)")
     ap.add_argument("--nerf-form", default="both", choices=["both", "plain", "proposal"])
     ap.add_argument("--nerf-pool", type=int, default=1 << 20, help="ground-truth rays the batches are drawn from")
     ap.add_argument("--image-iters", type=int, default=2000)

@@ -19,7 +19,8 @@
 // accumulation registers across the wave's tiles; each wave writes one partial
 // row, reduced afterwards in a fixed order.  Table gradients are staged in LDS
 // and scattered so that every atomic instruction carries four full 64-byte
-// lines (16 lanes on 16 consecutive channels of one texel).
+// lines (16 lanes on 16 consecutive channels of one texel); consecutive pixels
+// on the same texel and corner are summed first and share one atomic.
 #include "ff_field_impl.h"
 
 namespace ff {
@@ -96,7 +97,34 @@ struct DictCorners {
   float bw[6][kDPx][4];
   int ctex[kDPx][8];
   float cw[kDPx][8];
+  int head[2 * 64];  // run heads of the table being scattered
 };
+
+// Run heads of one table's (pixel, corner) pairs for corners kofs .. kofs+3,
+// lane = 4·pixel + corner.  A pair is live when its weight is nonzero; a live
+// pair whose predecessor (pixel - 1, same corner) is live on the same texel
+// continues that run.  Each run's first pair is appended to `head` at `base`
+// as pixel | corner << 4 | length << 8; returns the number of heads appended.
+// A batch ordered by image and coefficient cell puts pixels that share texels
+// next to each other, so one atomic then carries a whole run's sum; in any
+// other order every live pair is its own run.
+__device__ __forceinline__ int dict_runs(int t, float w, int kofs, int* head, int base) {
+  constexpr uint64_t kEvery4 = 0x1111111111111111ull;
+  const int lane = threadIdx.x & 63, px = lane >> 2, k = lane & 3;
+  const bool live = w != 0.f;
+  const int tp = __shfl_up(t, 4);
+  const int lp = __shfl_up((int)live, 4);
+  const bool member = live && lane >= 4 && lp != 0 && tp == t;
+  const bool is_head = live && !member;
+  const uint64_t M = __ballot(member);
+  const uint64_t H = __ballot(is_head);
+  // members following this pair on the same corner: bits 4j + k of M, j > px
+  const uint64_t after = px < kDPx - 1 ? ((M >> k) & kEvery4) >> (4 * (px + 1)) : 0ull;
+  const int len = 1 + __builtin_ctzll(~after & kEvery4) / 4;
+  const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(H >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)H, 0u));
+  if (is_head) head[base + pos] = px | ((k + kofs) << 4) | (len << 8);
+  return __popcll(H);
+}
 
 __device__ __forceinline__ void load_dict_weights(const ff_mlp_desc& m, DictW& W) {
   for (int e = threadIdx.x; e < 4 * 5 * 64 + kDMt * 2 * 64; e += blockDim.x) {
@@ -417,35 +445,34 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
         S.b.dc[r][c] = active ? df[mt][i] * P.bas[mt][i] : 0.f;
       }
     wave_lds_sync();
-    // Consecutive pixels whose corner k is the same texel (a batch ordered by
-    // pixel position shares basis texels) form a run: only the run's first
-    // pixel issues the atomic, with the run's sum.
 #pragma unroll
     for (int L = 0; L < 6; ++L) {
-      const int dl = ddim(L), nb = dl / 16, per = 4 * nb;
+      const int dl = ddim(L), nb = dl / 16;
       float* dst = g2.basis[L];
-      for (int it = g; it < kDPx * per; it += 4) {
-        const int px = it / per, rem = it - px * per, k = rem / nb, blk = rem - k * nb;
-        const int t = K.btex[L][px][k];
-        if (px > 0 && K.btex[L][px - 1][k] == t) continue;
-        const int c = doff(L) + 16 * blk + r;
+      const int nh = dict_runs(K.btex[L][lane >> 2][lane & 3], K.bw[L][lane >> 2][lane & 3], 0, K.head, 0);
+      wave_lds_sync();
+      for (int it = g; it < nh * nb; it += 4) {
+        const int e = K.head[nb == 2 ? it >> 1 : it], blk = nb == 2 ? it & 1 : 0;
+        const int px = e & 15, k = (e >> 4) & 7, len = e >> 8, c = doff(L) + 16 * blk + r;
         float v = 0.f;
-        for (int q = px; q < kDPx && K.btex[L][q][k] == t; ++q) v = fmaf(K.bw[L][q][k], S.b.db[q][c], v);
-        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)t * dl + 16 * blk + r, v);
+        for (int q = px; q < px + len; ++q) v = fmaf(K.bw[L][q][k], S.b.db[q][c], v);
+        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)K.btex[L][px][k] * dl + 16 * blk + r, v);
       }
+      wave_lds_sync();
     }
     {
       float* dst = g2.coeff[0];
-      for (int it = g; it < kDPx * 8 * kDMt; it += 4) {
-        const int px = it / (8 * kDMt), rem = it - px * 8 * kDMt, k = rem / kDMt, blk = rem - k * kDMt;
-        if (K.cw[px][k] == 0.f) continue;
-        const int t = K.ctex[px][k];
-        if (px > 0 && K.ctex[px - 1][k] == t && K.cw[px - 1][k] != 0.f) continue;
-        const int c = 16 * blk + r;
+      const int px = lane >> 2, k = lane & 3;
+      const int n0 = dict_runs(K.ctex[px][k], K.cw[px][k], 0, K.head, 0);
+      const int nh = n0 + dict_runs(K.ctex[px][k + 4], K.cw[px][k + 4], 4, K.head, n0);
+      wave_lds_sync();
+      for (int it = g; it < nh * kDMt; it += 4) {
+        const int hi = it / kDMt, blk = it - hi * kDMt;
+        const int e = K.head[hi];
+        const int p0 = e & 15, kk = (e >> 4) & 7, len = e >> 8, c = 16 * blk + r;
         float v = 0.f;
-        for (int q = px; q < kDPx && K.ctex[q][k] == t && K.cw[q][k] != 0.f; ++q)
-          v = fmaf(K.cw[q][k], S.b.dc[q][c], v);
-        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)t * kDC + 16 * blk + r, v);
+        for (int q = p0; q < p0 + len; ++q) v = fmaf(K.cw[q][kk], S.b.dc[q][c], v);
+        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)K.ctex[p0][kk] * kDC + c, v);
       }
     }
     wave_lds_sync();

@@ -148,6 +148,40 @@ def synth_image_set(n=1024, size=256, device="cuda", chunk=8):
     return out
 
 
+def cell_order(size, coeff_reso, device="cpu"):
+    """Pixel indices (row-major, [size*size]) ordered by the coefficient cell
+    whose four texels a pixel interpolates (cells of c = size // coeff_reso
+    pixels, offset by c/2 as the texel centres of align_corners=False are),
+    then by row and column inside the cell."""
+    c = max(size // coeff_reso, 1)
+    y, x = torch.meshgrid(torch.arange(size), torch.arange(size), indexing="ij")
+    y, x = y.reshape(-1), x.reshape(-1)
+    cy, cx = (y + c // 2) // c, (x + c // 2) // c
+    key = ((cy * (size + 1) + cx) * size + y) * size + x
+    return torch.argsort(key).to(device)
+
+
+def image_set_batch(n, n_images, size, generator, device, order=None):
+    """n (image, pixel) pairs drawn uniformly with replacement
+    (dataLoader/image_set.py:55-56).  Returns (img, pix) int64 tensors.
+
+    order=None: independent draws in draw order (randint, as the reference).
+    order=cell_order(...): the same distribution, delivered sorted by image
+    and coefficient cell.  The sorted values of n independent uniform draws
+    on [0, K) are floor(K * S_i / S_{n+1}) with S_i the partial sums of n + 1
+    exponential variates (order statistics of uniforms), so no sort runs."""
+    P = size * size
+    if order is None:
+        img = torch.randint(0, n_images, (n,), device=device, generator=generator)
+        pix = torch.randint(0, P, (n,), device=device, generator=generator)
+        return img, pix
+    K = n_images * P
+    e = torch.empty(n + 1, dtype=torch.float64, device=device).exponential_(generator=generator)
+    s = torch.cumsum(e, 0)
+    key = torch.floor(s[:n] * (K / s[n])).long().clamp_(0, K - 1)
+    return key // P, order[key % P]
+
+
 def dict_cfg(n_images=1024, size=256):
     """Dictionary image set: shared 2-D basis [32,32,32,16,16,16]@[32..128]
     (144 channels, sawtooth), per-image 32x32x144 coefficient grids, decoder

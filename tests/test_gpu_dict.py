@@ -69,8 +69,9 @@ def test_dict_forward():
 
 def repeated_points(n, seed, W=8, H=6, D=5):
     """Pixels drawn from a few pixel centres across all images, ordered by
-    position: consecutive pixels share basis texels (and some coefficient
-    texels), so the scatter's run merging is exercised."""
+    position: consecutive pixels of a wave share basis texels (and some
+    coefficient texels), so many lanes of one atomic instruction add into the
+    same addresses."""
     g = torch.Generator().manual_seed(seed)
     pos = torch.tensor([[0.5, 0.5], [3.5, 2.5], [7.5, 5.5], [4.5, 1.5]])
     k = torch.sort(torch.randint(0, len(pos), (n,), generator=g)).values
@@ -78,11 +79,26 @@ def repeated_points(n, seed, W=8, H=6, D=5):
     return torch.cat([pos[k], img], 1)
 
 
-@pytest.mark.parametrize("how", ["regression_mse", "autograd", "regression_runs"])
+def cell_sorted_points(n, seed):
+    """image_points ordered by image, then by the 2x2-pixel coefficient cell
+    (the 8x6 images on a 4-texel-wide coefficient grid), then by position:
+    runs of pixels sharing some or all coefficient corners, of varying length."""
+    x = image_points(n, seed)
+    cx, cy = torch.floor((x[:, 0] + 1) / 2), torch.floor((x[:, 1] + 1) / 2)
+    key = ((x[:, 2].floor() * 8 + cy) * 8 + cx) * 64 + x[:, 1] * 8 + x[:, 0]
+    return x[torch.argsort(key)].contiguous()
+
+
+@pytest.mark.parametrize("how", ["regression_mse", "autograd", "regression_runs", "regression_sorted"])
 def test_dict_backward(how):
     m = make()
     n = 2503
-    x = repeated_points(n, seed=5) if how == "regression_runs" else image_points(n, seed=5)
+    if how == "regression_runs":
+        x = repeated_points(n, seed=5)
+    elif how == "regression_sorted":
+        x = cell_sorted_points(n, seed=5)
+    else:
+        x = image_points(n, seed=5)
     tgt = torch.rand((n, 3), generator=torch.Generator().manual_seed(6))
     r = torch.randn((n, 3), generator=torch.Generator().manual_seed(7))
     m.zero_grad(set_to_none=True)

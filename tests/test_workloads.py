@@ -19,3 +19,46 @@ def test_dict_cfg_parameter_count():
     basis = sum(d * r * r for d, r in zip([32, 32, 32, 16, 16, 16], W.RESOS))
     lin = 144 * 64 + 64 + 64 * 3
     assert m.n_parameters() == 1024 * 32 * 32 * 144 + basis + lin
+
+
+def test_cell_order_groups_coefficient_cells():
+    """cell_order is a permutation of the pixels in which every coefficient
+    cell (the pixels interpolating the same four texels: (p + 4) // 8 on a
+    256-pixel image with 32 texels) is one contiguous block."""
+    o = W.cell_order(256, 32)
+    assert torch.equal(torch.sort(o).values, torch.arange(256 * 256))
+    y, x = o // 256, o % 256
+    cell = ((y + 4) // 8) * 64 + (x + 4) // 8
+    assert bool((cell[1:] >= cell[:-1]).all())
+    # texel corners of align_corners=False (pixel centre p + 0.5 -> p / 8 - 7/16)
+    x0 = torch.clamp((x.double() + 0.5) / 8 - 0.5, 0, 31).floor()
+    assert bool(((x + 4) // 8 - x0).abs().le(1).all())
+
+
+def test_ordered_batch_distribution():
+    """The ordered batch is sorted by (image, cell) and has the distribution
+    of independent uniform draws: chi-square of the image and column
+    histograms within 5 standard deviations of their degrees of freedom, and
+    the mean key within 5 standard errors of (K - 1) / 2."""
+    o = W.cell_order(64, 8)
+    n, N, P = 1 << 18, 32, 64 * 64
+    g = torch.Generator().manual_seed(3)
+    img, pix = W.image_set_batch(n, N, 64, g, "cpu", order=o)
+    rank = torch.argsort(o)
+    key = img * P + rank[pix]
+    assert bool((key[1:] >= key[:-1]).all())
+    assert int(img.min()) >= 0 and int(img.max()) < N and int(pix.min()) >= 0 and int(pix.max()) < P
+    for h, k in [(torch.bincount(img, minlength=N), N), (torch.bincount(pix % 64, minlength=64), 64)]:
+        e = n / k
+        chi2 = float(((h.double() - e) ** 2 / e).sum())
+        assert abs(chi2 - (k - 1)) < 5 * (2 * (k - 1)) ** 0.5, chi2
+    K = N * P
+    flat = img.double() * P + pix.double()
+    se = ((K * K - 1) / 12 / n) ** 0.5
+    assert abs(float(flat.mean()) - (K - 1) / 2) < 5 * se
+    # unordered: the reference's independent draws
+    g2 = torch.Generator().manual_seed(3)
+    i2, p2 = W.image_set_batch(16, N, 64, g2, "cpu")
+    g3 = torch.Generator().manual_seed(3)
+    assert torch.equal(i2, torch.randint(0, N, (16,), generator=g3))
+    assert torch.equal(p2, torch.randint(0, P, (16,), generator=g3))
