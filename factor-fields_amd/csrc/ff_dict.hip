@@ -92,29 +92,27 @@ union DictWave {
 };
 // Corners of the wave's pixels, written during the gather and read by the
 // scatter (not aliased with the areas above).
-struct DictCorners {
-  int btex[6][kDPx][4];
-  float bw[6][kDPx][4];
-  int ctex[kDPx][8];
-  float cw[kDPx][8];
-  int head[2 * 64];  // run heads of the table being scattered
+struct __attribute__((aligned(16))) DictCorners {
+  int2 bc[6][kDPx][4];  // basis corners: (texel, weight bits)
+  int2 cc[kDPx][8];     // coefficient corners
+  int bhead[6][64];     // run heads per basis level
+  int chead[2 * 64 + 4];  // run heads of the coefficient (corners 0-3, then 4-7), padded for pair reads
 };
 
 // Run heads of one table's (pixel, corner) pairs for corners kofs .. kofs+3,
-// lane = 4·pixel + corner.  A pair is live when its weight is nonzero; a live
-// pair whose predecessor (pixel - 1, same corner) is live on the same texel
-// continues that run.  Each run's first pair is appended to `head` at `base`
-// as pixel | corner << 4 | length << 8; returns the number of heads appended.
+// lane = 4·pixel + corner, given the lane's pair (t, w) and the previous
+// pixel's pair on the same corner (tp, wp).  A pair is live when its weight is
+// nonzero; a live pair whose predecessor is live on the same texel continues
+// that run.  Each run's first pair is appended to `head` at `base` as
+// pixel | corner << 4 | length << 8; returns the number of heads appended.
 // A batch ordered by image and coefficient cell puts pixels that share texels
 // next to each other, so one atomic then carries a whole run's sum; in any
 // other order every live pair is its own run.
-__device__ __forceinline__ int dict_runs(int t, float w, int kofs, int* head, int base) {
+__device__ __forceinline__ int dict_runs(int t, float w, int tp, float wp, int kofs, int* head, int base) {
   constexpr uint64_t kEvery4 = 0x1111111111111111ull;
   const int lane = threadIdx.x & 63, px = lane >> 2, k = lane & 3;
   const bool live = w != 0.f;
-  const int tp = __shfl_up(t, 4);
-  const int lp = __shfl_up((int)live, 4);
-  const bool member = live && lane >= 4 && lp != 0 && tp == t;
+  const bool member = live && px > 0 && wp != 0.f && tp == t;
   const bool is_head = live && !member;
   const uint64_t M = __ballot(member);
   const uint64_t H = __ballot(is_head);
@@ -171,8 +169,7 @@ __device__ __forceinline__ void dict_gather(const ff_field_desc& fd, const float
     if (K && g == 0) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        K->btex[L][px][k] = bc.tex[k];
-        K->bw[L][px][k] = bc.w[k];
+        K->bc[L][px][k] = make_int2(bc.tex[k], __float_as_int(bc.w[k]));
       }
     }
     const int dl = ddim(L);
@@ -201,8 +198,7 @@ __device__ __forceinline__ void dict_gather(const ff_field_desc& fd, const float
   if (K && g == 0) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      K->ctex[px][k] = cc.tex[k];
-      K->cw[px][k] = cc.w[k];
+      K->cc[px][k] = make_int2(cc.tex[k], __float_as_int(cc.w[k]));
     }
   }
 #pragma unroll
@@ -304,7 +300,7 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
                                                        float* __restrict__ loss_sum) {
   __shared__ DictW W;
   __shared__ __attribute__((aligned(16))) DictWave WV[kDWaves];
-  __shared__ DictCorners WK[kDWaves];
+  __shared__ __attribute__((aligned(16))) DictCorners WK[kDWaves];
   load_dict_weights(m, W);
   __syncthreads();
   const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4, l32 = lane & 31, h32 = lane >> 5;
@@ -445,34 +441,120 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
         S.b.dc[r][c] = active ? df[mt][i] * P.bas[mt][i] : 0.f;
       }
     wave_lds_sync();
+    // run heads of all seven tables in one pass (lane = 4 * pixel + corner)
+    int nbh[6], nch;
+    {
+      const int px = lane >> 2, k = lane & 3, pq = px > 0 ? px - 1 : 0;
+      int2 bp[6], bq[6];
+#pragma unroll
+      for (int L = 0; L < 6; ++L) {
+        bp[L] = K.bc[L][px][k];
+        bq[L] = K.bc[L][pq][k];
+      }
+      const int2 c0 = K.cc[px][k], c1 = K.cc[px][k + 4], q0 = K.cc[pq][k], q1 = K.cc[pq][k + 4];
+#pragma unroll
+      for (int L = 0; L < 6; ++L)
+        nbh[L] = dict_runs(bp[L].x, __int_as_float(bp[L].y), bq[L].x, __int_as_float(bq[L].y), 0, K.bhead[L], 0);
+      const int n0 = dict_runs(c0.x, __int_as_float(c0.y), q0.x, __int_as_float(q0.y), 0, K.chead, 0);
+      nch = n0 + dict_runs(c1.x, __int_as_float(c1.y), q1.x, __int_as_float(q1.y), 4, K.chead, n0);
+    }
+    wave_lds_sync();
+    // scatter: lane group g takes items 16p + 4g .. +3 (item = run head x
+    // 16-channel tile) in pass p; the heads come in one vector read and every
+    // LDS read of a pass is issued before its first use
 #pragma unroll
     for (int L = 0; L < 6; ++L) {
-      const int dl = ddim(L), nb = dl / 16;
+      const int dl = ddim(L), nb = dl / 16, items = nbh[L] * nb;
       float* dst = g2.basis[L];
-      const int nh = dict_runs(K.btex[L][lane >> 2][lane & 3], K.bw[L][lane >> 2][lane & 3], 0, K.head, 0);
-      wave_lds_sync();
-      for (int it = g; it < nh * nb; it += 4) {
-        const int e = K.head[nb == 2 ? it >> 1 : it], blk = nb == 2 ? it & 1 : 0;
-        const int px = e & 15, k = (e >> 4) & 7, len = e >> 8, c = doff(L) + 16 * blk + r;
-        float v = 0.f;
-        for (int q = px; q < px + len; ++q) v = fmaf(K.bw[L][q][k], S.b.db[q][c], v);
-        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)K.btex[L][px][k] * dl + 16 * blk + r, v);
+      for (int i0 = 4 * g; i0 < items; i0 += 16) {
+        int e[4];
+        if (nb == 2) {
+          const int2 h = *reinterpret_cast<const int2*>(&K.bhead[L][i0 >> 1]);
+          e[0] = h.x; e[1] = h.x; e[2] = h.y; e[3] = h.y;
+        } else {
+          const int4 h = *reinterpret_cast<const int4*>(&K.bhead[L][i0]);
+          e[0] = h.x; e[1] = h.y; e[2] = h.z; e[3] = h.w;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = i0 + u < items ? e[u] : 0;
+        int2 cp[4];
+        float dv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int px = e[u] & 15, k = (e[u] >> 4) & 7, blk = nb == 2 ? u & 1 : 0;
+          cp[u] = K.bc[L][px][k];
+          dv[u] = S.b.db[px][doff(L) + 16 * blk + r];
+        }
+        float v[4];
+        int lmax = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          v[u] = __int_as_float(cp[u].y) * dv[u];
+          lmax = max(lmax, e[u] >> 8);
+        }
+        // the rest of the four runs in lockstep (one LDS round trip per step)
+        for (int j = 1; j < lmax; ++j) {
+          float wq[4], dq[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int px = e[u] & 15, k = (e[u] >> 4) & 7, q = j < (e[u] >> 8) ? px + j : px;
+            wq[u] = __int_as_float(K.bc[L][q][k].y);
+            dq[u] = S.b.db[q][doff(L) + 16 * (nb == 2 ? u & 1 : 0) + r];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (j < (e[u] >> 8)) v[u] = fmaf(wq[u], dq[u], v[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (i0 + u < items && v[u] != 0.f)
+            unsafeAtomicAdd(dst + (int64_t)cp[u].x * dl + 16 * (nb == 2 ? u & 1 : 0) + r, v[u]);
       }
-      wave_lds_sync();
     }
     {
       float* dst = g2.coeff[0];
-      const int px = lane >> 2, k = lane & 3;
-      const int n0 = dict_runs(K.ctex[px][k], K.cw[px][k], 0, K.head, 0);
-      const int nh = n0 + dict_runs(K.ctex[px][k + 4], K.cw[px][k + 4], 4, K.head, n0);
-      wave_lds_sync();
-      for (int it = g; it < nh * kDMt; it += 4) {
-        const int hi = it / kDMt, blk = it - hi * kDMt;
-        const int e = K.head[hi];
-        const int p0 = e & 15, kk = (e >> 4) & 7, len = e >> 8, c = 16 * blk + r;
-        float v = 0.f;
-        for (int q = p0; q < p0 + len; ++q) v = fmaf(K.cw[q][kk], S.b.dc[q][c], v);
-        if (v != 0.f) unsafeAtomicAdd(dst + (int64_t)K.ctex[p0][kk] * kDC + c, v);
+      const int items = nch * kDMt;
+      for (int i0 = 4 * g; i0 < items; i0 += 16) {
+        const int h0 = i0 / kDMt;
+        const int2 hp = *reinterpret_cast<const int2*>(&K.chead[h0 & ~1]);
+        const int ha = (h0 & 1) ? hp.y : hp.x, hb = (h0 & 1) ? K.chead[h0 + 1] : hp.y;
+        int e[4], c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int t = i0 + u - h0 * kDMt;  // 0 .. 11: tile within head h0, or h0 + 1 past 8
+          e[u] = i0 + u < items ? (t < kDMt ? ha : hb) : 0;
+          c[u] = 16 * (t < kDMt ? t : t - kDMt) + r;
+        }
+        int2 cp[4];
+        float dv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int px = e[u] & 15, k = (e[u] >> 4) & 7;
+          cp[u] = K.cc[px][k];
+          dv[u] = S.b.dc[px][c[u]];
+        }
+        float v[4];
+        int lmax = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          v[u] = __int_as_float(cp[u].y) * dv[u];
+          lmax = max(lmax, e[u] >> 8);
+        }
+        for (int j = 1; j < lmax; ++j) {
+          float wq[4], dq[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int px = e[u] & 15, k = (e[u] >> 4) & 7, q = j < (e[u] >> 8) ? px + j : px;
+            wq[u] = __int_as_float(K.cc[q][k].y);
+            dq[u] = S.b.dc[q][c[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (j < (e[u] >> 8)) v[u] = fmaf(wq[u], dq[u], v[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (i0 + u < items && v[u] != 0.f) unsafeAtomicAdd(dst + (int64_t)cp[u].x * kDC + c[u], v[u]);
       }
     }
     wave_lds_sync();
