@@ -159,16 +159,35 @@ __device__ __forceinline__ void coding_fwd(const ff_field_desc& fd, const float*
 // (texel, weight) and gradient vector in a per-wave LDS area; for each corner
 // slot k, consecutive points whose corner k is the same texel (samples along
 // a ray, neighbouring query points) form a run, and only the first point of
-// each run enters a head list (wave prefix sum of per-point head counts).
+// each run enters a head list, in point-major order, with the run's length.
 // Groups of d consecutive lanes then take one head each, sum w * grad over its
 // run in registers and add the texel's d contiguous channels with one atomic
 // per channel.  Staging layout (words): G [64*d] | W [64*K] | T [64*K] |
 // heads [64*K].  Every lane of the wave must call wave_scatter, with the same
-// K and d.
+// K and d, from a block of one wave (the staging area belongs to the wave).
 constexpr int kStageWords = 64 * 18 + 64 * 8 * 3;
 
+// Orders the wave's LDS accesses across a hand-off between its lanes: a
+// wave's LDS operations execute in order, so waiting for its outstanding LDS
+// traffic and keeping the compiler from moving memory accesses across is
+// enough.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// Two loop forms with the same sums (each run added in point order):
+// - serial: the prefix sum of head counts by shuffles; each pass of heads walks
+//   its runs comparing texels; block barriers around the loop (their release
+//   fence also waits for the wave's outstanding atomics); fastest measured in
+//   the FP32 render backward;
+// - batched: heads from per-slot ballots and mbcnt, each entry carrying its run
+//   length; four passes of heads at a time with their LDS reads issued
+//   together and the runs summed in lockstep; fastest measured in the BF16
+//   matrix-core render backward and the proposal backward.
 template <int K, int d>
-__device__ __forceinline__ void wave_scatter(float* stage, float* dst, const int* tex, const float* w,
+__device__ __forceinline__ void wave_scatter_serial(float* stage, float* dst, const int* tex, const float* w,
                                              const float* g, bool active) {
   static_assert(d <= 64 && K <= 16 && 64 * d + 3 * 64 * K <= kStageWords, "staging area too small");
   const int lane = threadIdx.x & 63;
@@ -222,8 +241,91 @@ __device__ __forceinline__ void wave_scatter(float* stage, float* dst, const int
   __syncthreads();
 }
 
+template <int K, int d>
+__device__ __forceinline__ void wave_scatter_batched(float* stage, float* dst, const int* tex, const float* w,
+                                             const float* g, bool active) {
+  static_assert(d <= 64 && K <= 16 && 64 * d + 3 * 64 * K <= kStageWords, "staging area too small");
+  constexpr int U = 4;         // passes of heads handled together
+  constexpr int PPI = 64 / d;  // heads per pass
+  const int lane = threadIdx.x & 63;
+  float* G = stage;
+  float* W = stage + 64 * d;
+  int* T = reinterpret_cast<int*>(stage + 64 * d + 64 * K);
+  int* H = T + 64 * K;
+#pragma unroll
+  for (int c = 0; c < d; ++c) G[lane * d + c] = active ? g[c] : 0.f;
+  // run heads: per corner slot, ballots of heads and of run members; a head's
+  // position is the number of heads of all slots in earlier lanes (mbcnt)
+  // plus its own earlier slots; its length is 1 + the members right after it
+  uint32_t pos = 0;
+  int nh = 0, own = 0;
+  uint64_t hk[K];
+  int len[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int t = active ? tex[k] : -1;  // inactive lanes: a texel no run shares
+    W[lane * K + k] = active ? w[k] : 0.f;
+    T[lane * K + k] = t;
+    const int tp = __shfl_up(t, 1);
+    const bool member = t >= 0 && lane > 0 && t == tp;
+    const uint64_t M = __ballot(member);
+    hk[k] = __ballot(t >= 0 && !member);
+    const uint64_t after = lane < 63 ? M >> (lane + 1) : 0ull;
+    len[k] = 1 + __builtin_ctzll(~after);
+    pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(hk[k] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hk[k], pos));
+    nh += __popcll(hk[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((hk[k] >> lane) & 1) H[pos + own++] = lane | (k << 6) | (len[k] << 10);
+  wave_lds_fence();
+  const int sub = lane / d, c = lane - sub * d;
+  for (int base = 0; base < nh; base += U * PPI) {
+    int e[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int h = base + u * PPI + sub;
+      const bool ok = sub < PPI && h < nh;
+      const int v = H[ok ? h : 0];
+      e[u] = ok ? v : 0;  // 0: no head (length 0)
+    }
+    float acc[U];
+    int t[U], lmax = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s0 = e[u] & 63, k = (e[u] >> 6) & 15;
+      acc[u] = fmaf(W[s0 * K + k], G[s0 * d + c], 0.f);
+      t[u] = T[s0 * K + k];
+      lmax = max(lmax, e[u] >> 10);
+    }
+    for (int j = 1; j < lmax; ++j) {
+      float wq[U], gq[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int s0 = e[u] & 63, k = (e[u] >> 6) & 15, sq = j < (e[u] >> 10) ? s0 + j : s0;
+        wq[u] = W[sq * K + k];
+        gq[u] = G[sq * d + c];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j < (e[u] >> 10)) acc[u] = fmaf(wq[u], gq[u], acc[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if ((e[u] >> 10) > 0 && acc[u] != 0.f) unsafeAtomicAdd(dst + (int64_t)t[u] * d + c, acc[u]);
+  }
+  wave_lds_fence();
+}
+
+template <int K, int d, bool kBatched = false>
+__device__ __forceinline__ void wave_scatter(float* stage, float* dst, const int* tex, const float* w,
+                                             const float* g, bool active) {
+  if constexpr (kBatched) wave_scatter_batched<K, d>(stage, dst, tex, w, g, active);
+  else wave_scatter_serial<K, d>(stage, dst, tex, w, g, active);
+}
+
 // Level l of the basis: corners and scatter of grad[dl] (wave-wide).
-template <int D, int dl>
+template <int D, int dl, bool kB = false>
 __device__ __forceinline__ void basis_level_bwd_wave(const ff_field_desc& fd, float* dst, int l, const float* x,
                                                      const float* grad, bool active, float* stage) {
   const int R = fd.basis_reso[l];
@@ -237,17 +339,17 @@ __device__ __forceinline__ void basis_level_bwd_wave(const ff_field_desc& fd, fl
   if (fd.basis_interp == FF_INTERP_BILINEAR) {
     Corners<D> cr;
     linear_corners<D>(pos, Rs, cr);
-    wave_scatter<(1 << D), dl>(stage, dst, cr.tex, cr.w, grad, active);
+    wave_scatter<(1 << D), dl, kB>(stage, dst, cr.tex, cr.w, grad, active);
   } else {
     const int t = nearest_texel<D>(pos, Rs);
     const int tex[1] = {t >= 0 ? t : 0};
     const float w[1] = {t >= 0 ? 1.f : 0.f};
-    wave_scatter<1, dl>(stage, dst, tex, w, grad, active);
+    wave_scatter<1, dl, kB>(stage, dst, tex, w, grad, active);
   }
 }
 
 // Border-padded C-channel table at normalised pts (wave-wide scatter).
-template <int ND, int C>
+template <int ND, int C, bool kB = false>
 __device__ __forceinline__ void grid_scatter_wave(float* dst, const int* R, const float* pts, int interp,
                                                   const float* grad, bool active, float* stage) {
   float pos[ND];
@@ -256,17 +358,18 @@ __device__ __forceinline__ void grid_scatter_wave(float* dst, const int* R, cons
   if (interp == FF_INTERP_BILINEAR) {
     Corners<ND> cr;
     linear_corners<ND>(pos, R, cr);
-    wave_scatter<(1 << ND), C>(stage, dst, cr.tex, cr.w, grad, active);
+    wave_scatter<(1 << ND), C, kB>(stage, dst, cr.tex, cr.w, grad, active);
   } else {
     const int t = nearest_texel<ND>(pos, R);
     const int tex[1] = {t >= 0 ? t : 0};
     const float w[1] = {t >= 0 ? 1.f : 0.f};
-    wave_scatter<1, C>(stage, dst, tex, w, grad, active);
+    wave_scatter<1, C, kB>(stage, dst, tex, w, grad, active);
   }
 }
 
-// coding_bwd for a whole wave (every lane calls it; inactive lanes add nothing).
-template <int D, int CK, bool BF, class S>
+// coding_bwd for a whole wave (every lane calls it; inactive lanes add nothing);
+// kB selects the batched scatter loop.
+template <int D, int CK, bool BF, class S, bool kB = false>
 __device__ __forceinline__ void coding_bwd_wave(const ff_field_desc& fd, const ff_grads& g, const float* x,
                                                 const float* dbas, const float* dcof, bool active,
                                                 float* stage) {
@@ -274,7 +377,7 @@ __device__ __forceinline__ void coding_bwd_wave(const ff_field_desc& fd, const f
   if (fd.basis_kind == FF_BASIS_GRID) {
     static_for<0, S::L>([&](auto LI) {
       constexpr int l = decltype(LI)::value;
-      basis_level_bwd_wave<D, S::dim(l)>(fd, g.basis[l], l, x, dbas + S::off(l), active, stage);
+      basis_level_bwd_wave<D, S::dim(l), kB>(fd, g.basis[l], l, x, dbas + S::off(l), active, stage);
     });
   }
   constexpr int XD = XDim<D, CK>::value;
@@ -282,7 +385,7 @@ __device__ __forceinline__ void coding_bwd_wave(const ff_field_desc& fd, const f
     constexpr int ND = CK == CK_GRID2 ? 2 : 3;
     float pts[XD];
     coeff_pts<XD>(fd, x, pts);
-    grid_scatter_wave<ND, C>(g.coeff[0], fd.coeff_reso, pts, fd.coeff_interp, dcof, active, stage);
+    grid_scatter_wave<ND, C, kB>(g.coeff[0], fd.coeff_reso, pts, fd.coeff_interp, dcof, active, stage);
   } else if constexpr (CK == CK_VM) {
     float pts[3];
     coeff_pts<3>(fd, x, pts);
@@ -295,12 +398,12 @@ __device__ __forceinline__ void coding_bwd_wave(const ff_field_desc& fd, const f
       const float pp[2] = {pts[a0], pts[a1]};
 #pragma unroll
       for (int c = 0; c < C; ++c) t[c] = dcof[c] * lv[c];
-      grid_scatter_wave<2, C>(g.coeff[m], Rp, pp, fd.coeff_interp, t, active, stage);
+      grid_scatter_wave<2, C, kB>(g.coeff[m], Rp, pp, fd.coeff_interp, t, active, stage);
       const int Rl[2] = {1, fd.coeff_reso[av]};
       const float pl[2] = {0.f, pts[av]};
 #pragma unroll
       for (int c = 0; c < C; ++c) t[c] = dcof[c] * pv[c];
-      grid_scatter_wave<2, C>(g.coeff[3 + m], Rl, pl, fd.coeff_interp, t, active, stage);
+      grid_scatter_wave<2, C, kB>(g.coeff[3 + m], Rl, pl, fd.coeff_interp, t, active, stage);
     }
   }
 }

@@ -320,7 +320,7 @@ __global__ __launch_bounds__(64) void proposal_bwd_kernel(ff_proposal_desc p, co
     }
     __syncthreads();
     // grid scatter (all 64 lanes hold real proposal samples of a hit ray)
-    wave_scatter<8, kPC>(lds, grid_g, st.cr.tex, st.cr.w, df, true);
+    wave_scatter<8, kPC, true>(lds, grid_g, st.cr.tex, st.cr.w, df, true);
   }
   float* row = ws + wave * kPropParams;
   const int base = hi * kPC + half * 4;

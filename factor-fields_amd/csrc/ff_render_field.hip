@@ -237,7 +237,7 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_mfma_kernel(ff_field_desc f
       cof[k] = vals[C + k];
     }
     combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
-    coding_bwd_wave<3, CK, true, Spec18>(fd, gl, xv, db, dc, active, reinterpret_cast<float*>(lds));
+    coding_bwd_wave<3, CK, true, Spec18, true>(fd, gl, xv, db, dc, active, reinterpret_cast<float*>(lds));
   }
   acc.store(ws + (int64_t)blockIdx.x * (kMfH * kMfIn + kMfH + kMfO * kMfH));
 }

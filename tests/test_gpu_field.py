@@ -154,8 +154,11 @@ BF16_CASES = {
 
 @pytest.mark.parametrize("name", list(BF16_CASES))
 def test_bf16_tables_forward(name):
-    """BF16 tables: same result as the oracle evaluated on the BF16-rounded
-    tables (FP32 arithmetic)."""
+    """BF16 tables: features equal the oracle's on the BF16-rounded tables
+    (FP32 arithmetic).  The field output does too, except on the dictionary
+    layout, whose fused kernel runs the decoder on BF16 matrix cores: there it
+    is checked against the emulation of that decoder's rounding
+    (test_gpu_dict.emulate, same bars as there)."""
     print(f"[case] bf16 {name}", file=sys.stderr, flush=True)
     cfg = small_cfg(**BF16_CASES[name])
     m = new_model(cfg, dtype="bf16")
@@ -164,7 +167,13 @@ def test_bf16_tables_forward(name):
     feats, coeff = m.get_coding(x.cuda())
     rf, rc = of.get_coding(x)
     assert_close(feats, rf, FWD_RTOL, "feats")
-    assert_close(m.field(x.cuda()), of.field(x), FWD_RTOL, "field")
+    if name == "images_dict_c144":
+        from test_gpu_dict import emulate
+        assert m.fused_supported()
+        ref, *_ = emulate(m, rf)
+        assert_close(m.field(x.cuda()), ref, 2e-3, "field", atol_frac=2e-3)
+    else:
+        assert_close(m.field(x.cuda()), of.field(x), FWD_RTOL, "field")
 
 
 def test_empty_batch():
