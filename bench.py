@@ -150,15 +150,19 @@ def nerf_setup(dev, pool_rays=1 << 20, seed_off=0):
     return scene, rays, rgb
 
 
-def nerf_bytes_per_sample(model, table_bytes):
-    """Algorithmic HBM bytes per evaluated sample of nerf_field_bwd_kernel:
+def nerf_bytes_per_sample(model, table_bytes, stored_coding=False):
+    """Algorithmic HBM bytes per evaluated sample of the field backward:
     position (12 B) + direction (12 B) + the sample's upstream gradient
-    (sigma, rgb: 16 B) + one read of every gathered table value
-    (`table_bytes` each) + read-modify-write of its FP32 gradient (8 B each).
-    G = 8 corners per basis channel + 8 corners per coefficient channel."""
+    (sigma, rgb: 16 B) + the sample's coding + read-modify-write of every
+    gathered value's FP32 gradient (8 B each).  The coding is one read of
+    every gathered table value (`table_bytes` each) for the FP32 kernel, or,
+    with stored_coding (BF16 tables), the 2C FP32 basis and coefficient values
+    the forward kept for the sample.  G = 8 corners per basis channel + 8
+    corners per coefficient channel."""
     C = sum(model.basis_dims)
     G = 8 * C + 8 * C
-    return 12 + 12 + 16 + table_bytes * G + 8 * G, G
+    coding = 4 * 2 * C if stored_coding else table_bytes * G
+    return 12 + 12 + 16 + coding + 8 * G, G
 
 
 def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pretrain, check_rays=256):
@@ -242,7 +246,7 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
         out = [model.render_eval(test_rays[k:k + 4096], use_proposal=proposal_on(), N_samples=n_samples)
                for k in range(0, test_rays.shape[0], 4096)]
         mse = torch.mean((torch.cat(out) - test_rgb) ** 2).item()
-    bps, G = nerf_bytes_per_sample(model, 2 if prop else 4)
+    bps, G = nerf_bytes_per_sample(model, 2 if prop else 4, stored_coding=prop)
     samples_per_launch = samples / max(timed_steps, 1)
     return {"rays_per_s": timed_steps * B / dt, "field_pts_per_s": samples / dt,
             "ms_per_step": dt / timed_steps * 1e3, "samples_per_ray": samples / (timed_steps * B),

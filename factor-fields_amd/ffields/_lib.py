@@ -67,7 +67,7 @@ class RaySampler(ctypes.Structure):
 
 class Samples(ctypes.Structure):
     _fields_ = [("xyz", c_vp), ("z", c_vp), ("dist", c_vp), ("ray", c_vp), ("idx", c_vp),
-                ("ray_off", c_vp), ("cap", ctypes.c_int64)]
+                ("ray_off", c_vp), ("cap", ctypes.c_int64), ("coding", c_vp)]
 
 
 class RenderHead(ctypes.Structure):

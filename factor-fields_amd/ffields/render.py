@@ -56,6 +56,8 @@ def _samples_struct(buf):
     s.idx = buf["idx"].data_ptr()
     s.ray_off = buf["ray_off"].data_ptr()
     s.cap = buf["xyz"].shape[0]
+    coding = buf.get("coding")
+    s.coding = coding.data_ptr() if coding is not None else None
     return s
 
 
@@ -136,7 +138,10 @@ class RenderMixin:
 
     def render_buffers(self, B, Ns):
         """Packed-sample and per-sample buffers for B rays x Ns candidates,
-        kept between steps (a few shapes are cached)."""
+        kept between steps (a few shapes are cached).  They include the
+        per-sample coding rows, so a field backward on these buffers reads
+        what the field forward on them wrote: call the backward right after
+        the forward of the same samples, with the same tables."""
         key = (B, Ns)
         cache = self.__dict__.setdefault("_rbufs", {})
         if key not in cache:
@@ -145,8 +150,12 @@ class RenderMixin:
             dev = self.device
             buf = alloc_samples(B, Ns, dev)
             cap = buf["xyz"].shape[0]
+            # with BF16 tables the field forward keeps each sample's basis and
+            # coefficient values here for the backward of the same pass
             buf.update(sigma=torch.empty(cap, device=dev), rgb=torch.empty((cap, 3), device=dev),
-                       d_sample=torch.empty((cap, 4), device=dev))
+                       d_sample=torch.empty((cap, 4), device=dev),
+                       coding=(torch.empty((2 * self._spec.C, cap), device=dev)
+                               if self.table_dtype == "bf16" else None))
             cache[key] = buf
         self._rbuf = (key, cache[key])
         return cache[key]

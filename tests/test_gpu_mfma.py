@@ -98,10 +98,16 @@ def test_mfma_decoder_forward():
     assert rel.median().item() < 1e-4, rel.median().item()
 
 
-def test_mfma_decoder_backward():
-    print("[case] mfma decoder backward", file=sys.stderr, flush=True)
+@pytest.mark.parametrize("stored", [True, False])
+def test_mfma_decoder_backward(stored):
+    """stored: the backward reads the coding rows the forward wrote into the
+    render buffers; otherwise (coding NULL) it gathers the tables again."""
+    print(f"[case] mfma decoder backward stored={stored}", file=sys.stderr, flush=True)
     m = make_bf16_model(seed=3)
     rays, buf, s = sample_batch(m, B=260, seed=21)
+    assert s.coding
+    if not stored:
+        s.coding = None
     M = int(buf["ray_off"][-1])
     g = torch.Generator().manual_seed(22)
     d_sample = torch.randn((s.cap, 4), generator=g) * 0.1
