@@ -45,17 +45,49 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamBatch b, float beta1, flo
   const int R = b.reps[t];
   const int64_t rs = b.rstride[t];
   const float w1 = 1.f - beta1, w2 = 1.f - beta2;
+  auto update = [&](float g, float& m, float& v, float& p) {
+    m = m + w1 * (g - m);
+    v = v * beta2;
+    v = v + w2 * g * g;
+    const float denom = sqrtf(v) / bc2_sqrt + eps;
+    p = p + (-ss) * m / denom;
+  };
+  // one gradient copy, whole chunk in range, 16-byte aligned rows: four
+  // elements per lane per pass, the loads of a pass issued together
+  const bool vec = R == 1 && base + kAdamChunk <= n &&
+                   ((reinterpret_cast<uintptr_t>(P) | reinterpret_cast<uintptr_t>(G) |
+                     reinterpret_cast<uintptr_t>(M) | reinterpret_cast<uintptr_t>(V)) & 15) == 0 &&
+                   (!S || (reinterpret_cast<uintptr_t>(S) & 7) == 0);
+  if (vec) {
+    for (int k = 4 * threadIdx.x; k < kAdamChunk; k += 4 * blockDim.x) {
+      const int64_t i = base + k;
+      const float4 g4 = *reinterpret_cast<const float4*>(G + i);
+      float4 m4 = *reinterpret_cast<const float4*>(M + i);
+      float4 v4 = *reinterpret_cast<const float4*>(V + i);
+      float4 p4 = *reinterpret_cast<const float4*>(P + i);
+      update(g4.x, m4.x, v4.x, p4.x);
+      update(g4.y, m4.y, v4.y, p4.y);
+      update(g4.z, m4.z, v4.z, p4.z);
+      update(g4.w, m4.w, v4.w, p4.w);
+      *reinterpret_cast<float4*>(M + i) = m4;
+      *reinterpret_cast<float4*>(V + i) = v4;
+      *reinterpret_cast<float4*>(P + i) = p4;
+      if (S) {
+        const uint32_t lo = (uint32_t)f32_to_bf16(p4.x) | ((uint32_t)f32_to_bf16(p4.y) << 16);
+        const uint32_t hi = (uint32_t)f32_to_bf16(p4.z) | ((uint32_t)f32_to_bf16(p4.w) << 16);
+        *reinterpret_cast<uint2*>(S + i) = make_uint2(lo, hi);
+      }
+      if (zero_grad) *reinterpret_cast<float4*>(G + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    return;
+  }
   for (int k = threadIdx.x; k < kAdamChunk; k += blockDim.x) {
     const int64_t i = base + k;
     if (i >= n) break;
     float g = G[i];
     for (int r = 1; r < R; ++r) g += G[i + r * rs];
-    float m = M[i];
-    m = m + w1 * (g - m);
-    float v = V[i] * beta2;
-    v = v + w2 * g * g;
-    const float denom = sqrtf(v) / bc2_sqrt + eps;
-    const float p = P[i] + (-ss) * m / denom;
+    float m = M[i], v = V[i], p = P[i];
+    update(g, m, v, p);
     M[i] = m;
     V[i] = v;
     P[i] = p;
