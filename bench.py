@@ -174,7 +174,8 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
     kernels (FP32 masters updated by Adam), the learned proposal placing 48
     fine samples per ray and trained on the field's rendering weights each
     step; once the occupancy grid exists, the occupancy sampler renders the
-    first `check_rays` rays of every batch beside the proposal and the
+    first `check_rays` rays of every batch (side stream), their error is
+    compared with the proposal's training render of the same rays, and the
     proposal is switched off (training falls back to the occupancy sampler)
     if the accumulated PSNR gap exceeds 0.1 dB."""
     import torch
@@ -207,8 +208,9 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
         idx = torch.randint(0, pool.shape[0], (B,), device=dev, generator=gen)
         rays, rgb = pool[idx], pool_rgb[idx]
         if proposal_on():
-            referee.compare(model, rays, rgb)          # side stream, overlaps the backward
+            referee.compare(model, rays, rgb)          # side stream, overlaps the training pass
             model.render_train_step_proposal(rays, rgb, white_bg=True, reduce=False)
+            referee.add_proposal(model.last_rgb_map(), rgb)
             referee.join()
         else:
             model.render_train_step(rays, rgb, white_bg=True, N_samples=n_samples, reduce=False)

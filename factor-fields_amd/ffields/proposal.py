@@ -8,6 +8,7 @@ import ctypes
 import math
 
 import torch
+import torch.nn.functional as F
 import torch.nn as nn
 
 from . import _lib as L
@@ -95,14 +96,17 @@ class ProposalReferee:
     """The occupancy sampler as fallback and referee of the learned proposal.
 
     ``compare(model, rays, target)`` renders the first ``n_rays`` rays of a
-    training batch with the occupancy sampler and with the proposal (same
-    field, deterministic draws) and accumulates both squared errors on the
-    device.  The renders only read tables and weights, so they run on a side
-    stream (default priority) and overlap the batch's backward pass;
-    ``join()`` makes the current stream wait for them (call it before the
-    optimizer writes the tables).  ``decide()`` reads the accumulated errors
-    (one host synchronisation) and switches the proposal off for good when
-    PSNR(occupancy sampler) - PSNR(proposal) exceeds ``max_gap_db``.
+    training batch with the occupancy sampler and accumulates its squared
+    error on the device.  The render only reads tables and weights, so it
+    runs on a side stream (default priority) beside the batch's training
+    pass.  ``add_proposal(rgb_map, target)`` accumulates the proposal's
+    squared error on the same rays from the training pass's own render
+    (its first ``n_rays`` rows), so the proposal side costs one reduction
+    instead of a second render.  ``join()`` makes the current stream wait for
+    the side stream (call it before the optimizer writes the tables).
+    ``decide()`` reads the accumulated errors (one host synchronisation) and
+    switches the proposal off for good when PSNR(occupancy sampler) -
+    PSNR(proposal) exceeds ``max_gap_db``.
     """
 
     def __init__(self, device, n_rays=256, max_gap_db=0.1, n_samples=-1):
@@ -125,14 +129,22 @@ class ProposalReferee:
         with torch.cuda.stream(self.stream):
             self.stream.wait_event(ready)
             occ = model.render_eval(jr, use_proposal=False, N_samples=self.n_samples)
-            prop = model.render_eval(jr, use_proposal=True)
-            self.sq_err[0] += ((occ - jt) ** 2).sum()
-            self.sq_err[1] += ((prop - jt) ** 2).sum()
+            self.sq_err[0:1].add_(F.mse_loss(occ, jt, reduction="sum"))
             self._done = torch.cuda.Event()
             self._done.record(self.stream)
         # the batch tensors are read on the side stream: keep their memory until it is done
         jr.record_stream(self.stream)
         jt.record_stream(self.stream)
+        self._pending = True
+
+    def add_proposal(self, rgb_map, target):
+        """The proposal's squared error on the rays ``compare`` rendered
+        (current stream); counted only when ``compare`` ran for this batch."""
+        if not getattr(self, "_pending", False):
+            return
+        self._pending = False
+        n = self.n_rays
+        self.sq_err[1:2].add_(F.mse_loss(rgb_map[:n], target[:n], reduction="sum"))
 
     def join(self):
         if self._done is not None:

@@ -286,6 +286,12 @@ class RenderMixin:
             self.reduce_grad_replicas()
         return loss_sum
 
+    def last_rgb_map(self):
+        """rgb_map [B, 3] of the most recent fused training pass
+        (render_train_step or render_train_step_proposal); the buffer is
+        reused by the next pass."""
+        return self._rbuf[1]["rgb_map"]
+
     @torch.no_grad()
     def render_eval(self, rays, white_bg=True, use_proposal=None, N_samples=-1):
         """Forward-only render of `rays` with the occupancy sampler or the

@@ -124,3 +124,23 @@ def test_dict_backward(how):
     assert_close(lin[0].bias.grad, dW0a[:, 144], 2e-3, "db0", atol_frac=2e-3)
     for k, (t, o) in enumerate(zip(list(m.basises) + list(m.coeffs), of.basises + of.coeffs)):
         assert_close(t.grad.contiguous(), o.grad, 2e-3, f"table {k}", atol_frac=2e-3)
+
+
+def test_dict_batch_order_invariance():
+    """The same pixels in draw order and ordered by image and coefficient
+    cell (the bench's batch order) give the same gradients, up to FP32
+    summation order (atomics and per-wave partial sums)."""
+    m = make()
+    n = 4099
+    x = image_points(n, seed=8)
+    tgt = torch.rand((n, 3), generator=torch.Generator().manual_seed(9))
+    cx, cy = torch.floor((x[:, 0] + 1) / 2), torch.floor((x[:, 1] + 1) / 2)
+    perm = torch.argsort(((x[:, 2].floor() * 8 + cy) * 8 + cx) * 64 + x[:, 1] * 8 + x[:, 0])
+    params = list(m.basises) + list(m.coeffs) + list(m.linear_mat.parameters())
+    grads = []
+    for xi, ti in [(x, tgt), (x[perm].contiguous(), tgt[perm].contiguous())]:
+        m.zero_grad(set_to_none=True)
+        m.regression_step(xi.cuda(), ti.cuda(), loss="mse")
+        grads.append([p.grad.detach().clone() for p in params])
+    for k, (a, b) in enumerate(zip(*grads)):
+        assert_close(a, b, 1e-4, f"param {k}", atol_frac=1e-5)

@@ -65,9 +65,10 @@ def test_trained_proposal_beats_uniform_placement():
 
 
 def test_referee_matches_same_stream_renders():
-    """ProposalReferee's side-stream renders give the same squared errors as
-    rendering on the current stream, and the fallback switch trips only above
-    the bar."""
+    """ProposalReferee's side-stream occupancy render gives the same squared
+    error as rendering on the current stream, the proposal's error is taken
+    from the render it is given (here an evaluation render), and the
+    fallback switch trips only above the bar."""
     from ffields.proposal import ProposalReferee
     from test_gpu_render import make_model, make_rays
     print("[case] proposal referee", file=sys.stderr, flush=True)
@@ -76,14 +77,37 @@ def test_referee_matches_same_stream_renders():
     rays = make_rays(300, seed=40).cuda()
     tgt = torch.rand((300, 3), generator=torch.Generator().manual_seed(41)).cuda()
     ref = ProposalReferee(rays.device, n_rays=200, max_gap_db=0.1)
+    prop_all = m.render_eval(rays, use_proposal=True)
+    ref.add_proposal(prop_all, tgt)            # no compare yet: not counted
+    assert ref.sq_err.abs().sum().item() == 0
     ref.compare(m, rays, tgt)
+    ref.add_proposal(prop_all, tgt)
     ref.join()
     got = ref.sq_err.clone()
     occ = m.render_eval(rays[:200], use_proposal=False)
-    prop = m.render_eval(rays[:200], use_proposal=True)
-    want = torch.stack([((occ - tgt[:200]) ** 2).sum(), ((prop - tgt[:200]) ** 2).sum()])
+    want = torch.stack([((occ - tgt[:200]) ** 2).sum(), ((prop_all[:200] - tgt[:200]) ** 2).sum()])
     assert torch.allclose(got, want, rtol=1e-6, atol=0), (got, want)
     gap = ref.decide()
     assert abs(gap - 10 * math.log10(want[1].item() / want[0].item())) < 1e-4
     assert ref.active == (gap <= 0.1)
     assert ref.sq_err.abs().sum().item() == 0
+
+
+def test_last_rgb_map_is_the_training_render():
+    """last_rgb_map() after a fused proposal training pass equals the same
+    pass's render of the batch (forward only, same draws)."""
+    from test_gpu_render import make_model, make_rays
+    print("[case] last rgb map", file=sys.stderr, flush=True)
+    m = make_model(seed=6)
+    m.enable_proposal(reso=16, n_fine=48, table_dtype="fp32")
+    B = 256
+    rays = make_rays(B, seed=50).cuda()
+    tgt = torch.rand((B, 3), generator=torch.Generator().manual_seed(51)).cuda()
+    g = torch.Generator().manual_seed(52)
+    pu, fu = torch.rand((B, 64), generator=g).cuda(), torch.rand((B, 48), generator=g).cuda()
+    m.render_train_step_proposal(rays, tgt, white_bg=True, prop_u=pu, fine_u=fu)
+    got = m.last_rgb_map().clone()
+    ls = torch.zeros((), device="cuda")
+    m.render_train_step_proposal(rays, tgt, white_bg=True, prop_u=pu, fine_u=fu, loss_sum=ls)
+    assert torch.equal(got, m.last_rgb_map())
+    assert abs(((got - tgt) ** 2).sum().item() - ls.item()) <= 1e-5 * ls.item()
