@@ -18,8 +18,11 @@ __global__ __launch_bounds__(256) void reduce_partials_stage1(const float* __res
   const int k = blockIdx.x * 64 + (threadIdx.x & 63);
   const int grp = threadIdx.x >> 6;
   float s = 0.f;
-  if (k < P)
+  if (k < P) {
+    // unrolled so the loads go out together; the sums keep their order
+#pragma unroll 8
     for (int r = blockIdx.y + kReduceSplit * grp; r < rows; r += 4 * kReduceSplit) s += ws[(int64_t)r * P + k];
+  }
   part[grp][threadIdx.x & 63] = s;
   __syncthreads();
   if (grp == 0 && k < P)
@@ -34,6 +37,7 @@ __global__ __launch_bounds__(256) void reduce_partials_stage2(const float* __res
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= P) return;
   float t = 0.f;
+#pragma unroll
   for (int y = 0; y < kReduceSplit; ++y) t += part2[(int64_t)y * P + k];
   if (k < n_w0) gw0[k] += t;
   else if (k < n_w0 + n_b0) gb0[k - n_w0] += t;
