@@ -54,14 +54,48 @@ __device__ __forceinline__ void basis_level_fwd(const ff_field_desc& fd, int l, 
 #pragma unroll
   for (int c = 0; c < dl; ++c) out[c] = 0.f;
   if (fd.basis_interp == FF_INTERP_BILINEAR) {
-    Corners<D> cr;
-    linear_corners<D>(pos, Rs, cr);
+    // The x-neighbour corners (k, k+1) of each (y, z) come from one load of
+    // two adjacent texels: the pair starts at xb = clamp(x0, 0, R-2) in the
+    // row (row indices clamped into the table).  Corners, weights (in the
+    // reference's product order) and the accumulation order are those of
+    // linear_corners; a corner outside the grid has weight 0, whichever
+    // texel of the pair it reads.
+    float f0[D], w0[D], w1[D];
+    int i0[D];
 #pragma unroll
-    for (int k = 0; k < (1 << D); ++k) {
-      float v[dl];
-      load_ch<dl, BF>(fd.basis[l], (int64_t)cr.tex[k] * dl, v);
+    for (int a = 0; a < D; ++a) {
+      const float p = safe_coord(pos[a], R);
+      f0[a] = floorf(p);
+      i0[a] = (int)f0[a];
+      w1[a] = p - f0[a];
+      w0[a] = (f0[a] + 1.f) - p;
+    }
+    const int xb = min(max(i0[0], 0), R - 2);
+    const bool sel = i0[0] == xb;  // x0 is the pair's first texel (else its second)
+    const bool okx0 = i0[0] >= 0 && i0[0] < R, okx1 = i0[0] + 1 >= 0 && i0[0] + 1 < R;
 #pragma unroll
-      for (int c = 0; c < dl; ++c) out[c] = fmaf(v[c], cr.w[k], out[c]);
+    for (int pr = 0; pr < (1 << (D - 1)); ++pr) {
+      int row = 0;
+      bool okr = true;
+#pragma unroll
+      for (int a = D - 1; a >= 1; --a) {
+        const int ii = i0[a] + ((pr >> (a - 1)) & 1);
+        okr = okr && ii >= 0 && ii < R;
+        row = row * R + min(max(ii, 0), R - 1);
+      }
+      float v2[2 * dl];
+      load_ch<2 * dl, BF>(fd.basis[l], ((int64_t)row * R + xb) * dl, v2);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = 2 * pr + h;
+        float w = h ? w1[0] : w0[0];
+#pragma unroll
+        for (int a = 1; a < D; ++a) w = w * (((k >> a) & 1) ? w1[a] : w0[a]);
+        w = (okr && (h ? okx1 : okx0)) ? w : 0.f;
+        const float* v = ((h == 0) == sel) ? v2 : v2 + dl;
+#pragma unroll
+        for (int c = 0; c < dl; ++c) out[c] = fmaf(v[c], w, out[c]);
+      }
     }
   } else {
     const int t = nearest_texel<D>(pos, Rs);
