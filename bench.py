@@ -414,11 +414,22 @@ def run_dict(args):
     return out
 
 
+def image_bytes_per_point(model, out_dim=3):
+    """Algorithmic HBM bytes per pixel of the fused 2-D regression kernel:
+    coordinates (8 B) + target (4 B per channel) + one read of every
+    gathered table value (4 B each) + read-modify-write of its FP32 gradient
+    (8 B each); G = 4 bilinear corners per basis and coefficient channel."""
+    C = sum(model.basis_dims)
+    G = 4 * C + 4 * C
+    return 8 + 4 * out_dim + 4 * G + 8 * G, G
+
+
 def run_image(args):
     """2D image regression: 512x512x3 synthetic target, 65,536 uniformly sampled
     pixels per step, 2-D grid basis [4,4,4,2,2,2]@[32..128] (sawtooth, literal
     freq bands), 64x64x18 coefficient grid, MLP 18-64-3, MSE, Adam lr 0.02,
-    2,000 steps; reports PSNR over the full image and points/s."""
+    2,000 steps (all timed); reports points/s, PSNR over the full image and the
+    fused training kernel's roofline."""
     import torch
     from ffields import FactorFields, FusedAdam
     from ffields import workloads as W
@@ -436,24 +447,43 @@ def run_image(args):
     gen = torch.Generator(device=dev).manual_seed(W.BASE_SEED + 3)
     iters = args.image_iters
 
-    def step():
+    def step(ev=None):
         idx = torch.randint(0, H * Wd, (n,), device=dev, generator=gen)
-        model.regression_step(coords[idx], rgb[idx], loss="mse", reduce=False)
+        x, t = coords[idx], rgb[idx]
+        if ev is not None:
+            ev[0].record()
+        model.regression_step(x, t, loss="mse", reduce=False)
+        if ev is not None:
+            ev[1].record()
         opt.step(zero_grad=True)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(iters)]
     t0 = time.perf_counter()
-    for _ in range(iters):
-        step()
+    for k in range(iters):
+        step(evs[k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / iters
     with torch.no_grad():
         out = model.field(coords).clamp(0, 1)
         mse = torch.mean((out - rgb) ** 2).item()
-    return {"pts_per_s": iters * n / dt, "ms_per_step": dt / iters * 1e3, "steps": iters,
-            "psnr": -10 * math.log10(mse), "params": model.n_parameters()}
+    bpp, G = image_bytes_per_point(model)
+    achieved = bpp * n / (kern_ms * 1e-3) / 1e9
+    return {
+        "metric": METRIC, "value": iters * n / dt, "unit": "field pts/s", "n_gpus": 1, "steps": iters,
+        "warmup": args.warmup, "ms_per_step": dt / iters * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Gaussian+sinusoid image, 512x512)",
+        "config": {"workload": "image_regression: 512x512x3, 65,536 px/step, 2-D grid basis [4,4,4,2,2,2]@[32..128] "
+                               "sawtooth, 64x64x18 coefficient, MLP 18-64-3, MSE, Adam 0.02",
+                   "psnr_full_image": -10 * math.log10(mse), "params": model.n_parameters()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "field_bwd_kernel (fused forward+loss+backward)",
+                     "kernel_ms": kern_ms, "bytes_per_point": bpp, "gathered_values_per_point": G},
+    }
 
 
 def cpu_baseline_sdf(args):
