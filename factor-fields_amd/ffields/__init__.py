@@ -1,8 +1,10 @@
 """ffields: MI355X-native (gfx950 HIP) Factor Fields field evaluation and
 volume rendering, behind the reference's FactorFields model API.
 
-Everything numerical runs in libffields.so (hand-written HIP kernels behind a
-C ABI, include/ffields.h); PyTorch provides device memory and streams.
+Every step of the field and render path runs in libffields.so (hand-written
+HIP kernels behind a C ABI, include/ffields.h); PyTorch provides device memory
+and streams.  The evaluation metrics (ffields.evaluation: PSNR, SSIM) are
+computed with torch on the rendered images.
 """
 
 from .config import AttrDict, load_config

@@ -1,0 +1,71 @@
+"""Evaluation metrics (renderer.py:30-100, utils.py:110-156): SSIM against
+its closed form on constant images and against a brute-force evaluation of
+every window with the explicit 2-D Gaussian weights; PSNR by definition."""
+import math
+
+import torch
+
+from ffields.evaluation import _gauss_taps, psnr, ssim
+
+
+def test_taps():
+    t = _gauss_taps(11, 1.5, torch.float64, "cpu")
+    assert abs(float(t.sum()) - 1.0) < 1e-15
+    assert torch.allclose(t, t.flip(0), rtol=0, atol=1e-18)
+    assert int(t.argmax()) == 5
+    assert abs(float(t[6] / t[5]) - math.exp(-0.5 / 1.5 ** 2)) < 1e-14
+
+
+def test_ssim_constant_images_closed_form():
+    """Constant images a, b have zero variances: SSIM = (2ab + c1) / (a^2 + b^2 + c1)."""
+    c1 = (0.01 * 1.0) ** 2
+    for a, b in [(0.3, 0.7), (0.0, 0.5), (0.9, 0.9)]:
+        x = torch.full((16, 20, 3), a, dtype=torch.float64)
+        y = torch.full((16, 20, 3), b, dtype=torch.float64)
+        want = (2 * a * b + c1) / (a * a + b * b + c1)
+        assert abs(ssim(x, y) - want) < 1e-12, (a, b)
+
+
+def brute_ssim(x, y, size=11, sigma=1.5, k1=0.01, k2=0.03, max_val=1.0):
+    t = _gauss_taps(size, sigma, torch.float64, "cpu")
+    w2 = t[:, None] * t[None, :]
+    H, W, C = x.shape
+    c1, c2 = (k1 * max_val) ** 2, (k2 * max_val) ** 2
+    vals = []
+    for c in range(C):
+        for i in range(H - size + 1):
+            for j in range(W - size + 1):
+                a = x[i:i + size, j:j + size, c]
+                b = y[i:i + size, j:j + size, c]
+                m0, m1 = (w2 * a).sum(), (w2 * b).sum()
+                v0 = max(float((w2 * a * a).sum() - m0 * m0), 0.0)
+                v1 = max(float((w2 * b * b).sum() - m1 * m1), 0.0)
+                cv = float((w2 * a * b).sum() - m0 * m1)
+                cv = math.copysign(min(math.sqrt(v0 * v1), abs(cv)), cv) if cv != 0 else 0.0
+                m0, m1 = float(m0), float(m1)
+                vals.append(((2 * m0 * m1 + c1) * (2 * cv + c2)) / ((m0 * m0 + m1 * m1 + c1) * (v0 + v1 + c2)))
+    return sum(vals) / len(vals)
+
+
+def test_ssim_matches_windowed_brute_force():
+    g = torch.Generator().manual_seed(0)
+    x = torch.rand((14, 13, 3), generator=g, dtype=torch.float64)
+    y = (x + 0.2 * torch.randn((14, 13, 3), generator=g, dtype=torch.float64)).clamp(0, 1)
+    assert abs(ssim(x, y) - brute_ssim(x, y)) < 1e-12
+    smap = ssim(x, y, return_map=True)
+    assert smap.shape == (4, 3, 3)
+
+
+def test_ssim_identity_and_symmetry():
+    g = torch.Generator().manual_seed(1)
+    x = torch.rand((24, 30, 3), generator=g)
+    y = torch.rand((24, 30, 3), generator=g)
+    assert abs(ssim(x, x) - 1.0) < 1e-12
+    assert abs(ssim(x, y) - ssim(y, x)) < 1e-12
+    assert ssim(x, y) < 0.2
+
+
+def test_psnr_definition():
+    x = torch.zeros((4, 5, 3), dtype=torch.float64)
+    y = torch.full((4, 5, 3), 0.1, dtype=torch.float64)
+    assert abs(psnr(x, y) - 20.0) < 1e-9

@@ -1,0 +1,32 @@
+"""evaluation() renders each test image through render_ray in chunks and
+scores it: its images equal a direct render of the same rays, and its PSNR
+and SSIM equal the metrics applied to those images."""
+import sys
+
+import pytest
+import torch
+
+from ffields.evaluation import evaluation, psnr, ssim
+from test_gpu_render import make_model, make_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def test_evaluation_matches_direct_render():
+    print("[case] evaluation", file=sys.stderr, flush=True)
+    m = make_model(seed=8)
+    W, H, n = 16, 12, 2
+    rays = make_rays(n * H * W, seed=60).reshape(n, H * W, 6)
+    gt = torch.rand((n, H * W, 3), generator=torch.Generator().manual_seed(61))
+    out = evaluation(rays, gt, m, (W, H), chunk=100, white_bg=True)
+    assert out["rgb_maps"].shape == (n, H, W, 3) and out["depth_maps"].shape == (n, H, W)
+    for i in range(n):
+        with torch.no_grad():
+            rgb, depth, _ = m(rays[i].cuda(), white_bg=True, is_train=False)
+        want = rgb.clamp(0, 1).reshape(H, W, 3).cpu()
+        assert torch.equal(out["rgb_maps"][i], want)
+        assert torch.equal(out["depth_maps"][i], depth.reshape(H, W).cpu())
+        assert out["psnr"][i] == psnr(want, gt[i].reshape(H, W, 3))
+        # float64 convolutions on the GPU and the CPU differ in the last digits
+        assert abs(out["ssim"][i] - ssim(want, gt[i].reshape(H, W, 3))) < 1e-12
+    assert abs(out["psnr_mean"] - sum(out["psnr"]) / n) < 1e-12
