@@ -8,12 +8,19 @@ window, the variances clipped at zero and the covariance clipped to
 +-sqrt(var0 * var1), with the constants (k1 * max)^2 and (k2 * max)^2.  It is
 computed here in float64 with torch, on the images' device.  LPIPS needs
 pretrained network weights that cannot be fetched offline; it is not
-computed."""
-import math
+computed.
 
+evaluation_path renders a camera path (renderer.py:101-155) and writes the
+frames as PNG files with PIL; the reference's MP4 videos need imageio, which
+is not installed, so no video is written."""
+import math
+import os
+
+import numpy as np
 import torch
 import torch.nn.functional as F
 
+from .rays import get_rays
 from .render import render_ray
 
 
@@ -89,3 +96,74 @@ def evaluation(rays, rgbs, model, img_wh, chunk=1024, N_samples=-1, white_bg=Fal
     if ssims:
         out["ssim_mean"] = sum(ssims) / len(ssims)
     return out
+
+
+def _jet_bgr():
+    """256-entry jet colour table in BGR channel order (the order of OpenCV's
+    colour maps), from matplotlib's piecewise-linear jet."""
+    from matplotlib import colormaps
+    rgb = colormaps["jet"](np.arange(256))[:, :3]
+    return np.round(rgb * 255).astype(np.uint8)[:, ::-1].copy()
+
+
+def visualize_depth_numpy(depth, minmax=None):
+    """Colour-mapped depth (utils.py:11-26): NaN -> 0, normalised by
+    (x - mi) / (ma - mi + 1e-8) with mi the smallest positive depth and ma the
+    largest (or `minmax`), cast to 8 bits as numpy casts, then looked up in a
+    jet table.  Returns ([H, W, 3] uint8 BGR, [mi, ma]).  The table is
+    matplotlib's jet; OpenCV's, which the reference uses, differs from it by a
+    few levels per channel."""
+    x = np.nan_to_num(np.asarray(depth))
+    if minmax is None:
+        mi = np.min(x[x > 0])
+        ma = np.max(x)
+    else:
+        mi, ma = minmax
+    x = (x - mi) / (ma - mi + 1e-8)
+    x = (255 * x).astype(np.uint8)
+    return _jet_bgr()[x], [mi, ma]
+
+
+@torch.no_grad()
+def evaluation_path(test_dataset, factor_fields, c2ws, renderer=render_ray, savePath=None, prtx="", N_samples=-1,
+                    white_bg=False, ndc_ray=False, compute_extra_metrics=True, device="cuda", chunk=8192,
+                    return_maps=False):
+    """Render the views of a camera path (renderer.py:101-155).
+
+    test_dataset supplies img_wh = (W, H), directions [H, W, 3] and near_far;
+    each c2w [3, 4] (or [4, 4]) gives the rays of one frame through get_rays.
+    Frames are clamped to [0, 1] and converted to 8 bits; depth is colour
+    mapped over near_far.  With savePath, frame k is written to
+    savePath/{prtx}{k:03d}.png and the frame beside its depth to
+    savePath/rgbd/{prtx}{k:03d}.png.  Returns the PSNR list, which is empty as
+    in the reference (a path has no ground truth), or with return_maps a dict
+    of that list and the uint8 ``rgb_maps`` [n, H, W, 3] and ``depth_maps``
+    [n, H, W, 3]."""
+    if ndc_ray:
+        raise NotImplementedError("NDC rays are not supported")
+    W, H = test_dataset.img_wh
+    if savePath is not None:
+        os.makedirs(savePath, exist_ok=True)
+        os.makedirs(os.path.join(savePath, "rgbd"), exist_ok=True)
+    near_far = test_dataset.near_far
+    PSNRs, rgb_maps, depth_maps = [], [], []
+    for idx, c2w in enumerate(c2ws):
+        c2w = torch.as_tensor(np.asarray(c2w), dtype=torch.float32)
+        rays_o, rays_d = get_rays(torch.as_tensor(test_dataset.directions, dtype=torch.float32), c2w)
+        rays = torch.cat([rays_o, rays_d], 1)
+        rgb_map, depth_map = renderer(rays, factor_fields, chunk=chunk, N_samples=N_samples, ndc_ray=ndc_ray,
+                                      white_bg=white_bg, device=device)
+        rgb_map = rgb_map.clamp(0.0, 1.0)
+        rgb_map, depth_map = rgb_map.reshape(H, W, 3).cpu(), depth_map.reshape(H, W).cpu()
+        depth_img, _ = visualize_depth_numpy(depth_map.numpy(), near_far)
+        rgb_img = (rgb_map.numpy() * 255).astype("uint8")
+        rgb_maps.append(rgb_img)
+        depth_maps.append(depth_img)
+        if savePath is not None:
+            from PIL import Image
+            Image.fromarray(rgb_img).save(os.path.join(savePath, f"{prtx}{idx:03d}.png"))
+            Image.fromarray(np.concatenate((rgb_img, depth_img), axis=1)).save(
+                os.path.join(savePath, "rgbd", f"{prtx}{idx:03d}.png"))
+    if return_maps:
+        return {"psnr": PSNRs, "rgb_maps": np.stack(rgb_maps), "depth_maps": np.stack(depth_maps)}
+    return PSNRs

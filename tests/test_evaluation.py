@@ -1,7 +1,11 @@
 """Evaluation metrics (renderer.py:30-100, utils.py:110-156): SSIM against
 its closed form on constant images and against a brute-force evaluation of
-every window with the explicit 2-D Gaussian weights; PSNR by definition."""
+every window with the explicit 2-D Gaussian weights; PSNR by definition.
+Depth colouring at levels fixed by its normalisation, and the frames and
+files of evaluation_path (renderer.py:101-155) with a stand-in renderer."""
 import math
+
+import numpy as np
 
 import torch
 
@@ -69,3 +73,59 @@ def test_psnr_definition():
     x = torch.zeros((4, 5, 3), dtype=torch.float64)
     y = torch.full((4, 5, 3), 0.1, dtype=torch.float64)
     assert abs(psnr(x, y) - 20.0) < 1e-9
+
+
+def test_visualize_depth_known_levels():
+    """Smallest positive depth -> level 0, the largest -> level 254 (255 * (1 -
+    1e-8 / range) truncated), the midpoint -> 127; colours from the jet table
+    in BGR order (level 0 is dark blue: only the first channel lit)."""
+    from ffields.evaluation import _jet_bgr, visualize_depth_numpy
+    jet = _jet_bgr()
+    assert jet[0, 0] > 100 and jet[0, 1] == 0 and jet[0, 2] == 0
+    assert jet[255, 2] > 100 and jet[255, 0] == 0 and jet[255, 1] == 0
+    d = np.array([[2.0, 4.0], [3.0, 3.0]])
+    img, mm = visualize_depth_numpy(d)
+    assert mm == [2.0, 4.0] and img.shape == (2, 2, 3) and img.dtype == np.uint8
+    assert (img[0, 0] == jet[0]).all() and (img[0, 1] == jet[254]).all() and (img[1, 0] == jet[127]).all()
+    img2, mm2 = visualize_depth_numpy(d, [0.0, 8.0])
+    assert mm2 == [0.0, 8.0] and (img2[0, 1] == jet[127]).all() and (img2[0, 0] == jet[63]).all()
+
+
+class _PathData:
+    def __init__(self, H=6, W=8):
+        from ffields.rays import get_ray_directions
+        self.img_wh = (W, H)
+        self.directions = get_ray_directions(H, W, (10.0, 10.0))
+        self.near_far = [2.0, 6.0]
+
+
+def _stub_renderer(rays, model, chunk, N_samples, ndc_ray, white_bg, device):
+    """Colour from the ray direction, depth from the origin's distance; checks
+    that the chunk size arrives."""
+    assert chunk == 8192 and not ndc_ray
+    d = rays[:, 3:6]
+    rgb = (d / d.norm(dim=1, keepdim=True) * 1.5 + 0.25)        # partly outside [0, 1]: clamped
+    return rgb, rays[:, :3].norm(dim=1)
+
+
+def test_evaluation_path_frames(tmp_path):
+    from PIL import Image
+
+    from ffields.evaluation import evaluation_path, visualize_depth_numpy
+    from ffields.rays import get_rays
+    data = _PathData()
+    c2ws = [torch.eye(4)[:3], torch.tensor([[0.0, 0, 1, 3], [0, 1, 0, 0], [-1, 0, 0, 1]])]
+    out = evaluation_path(data, None, c2ws, renderer=_stub_renderer, savePath=str(tmp_path), prtx="p_",
+                          device="cpu", return_maps=True)
+    assert out["psnr"] == [] and out["rgb_maps"].shape == (2, 6, 8, 3) and out["depth_maps"].shape == (2, 6, 8, 3)
+    for k, c2w in enumerate(c2ws):
+        o, d = get_rays(data.directions, c2w)
+        rgb, depth = _stub_renderer(torch.cat([o, d], 1), None, 8192, -1, False, False, "cpu")
+        want = (rgb.clamp(0, 1).reshape(6, 8, 3).numpy() * 255).astype(np.uint8)
+        assert (out["rgb_maps"][k] == want).all()
+        dimg, _ = visualize_depth_numpy(depth.reshape(6, 8).numpy(), data.near_far)
+        assert (out["depth_maps"][k] == dimg).all()
+        assert (np.asarray(Image.open(tmp_path / f"p_{k:03d}.png")) == want).all()
+        rgbd = np.asarray(Image.open(tmp_path / "rgbd" / f"p_{k:03d}.png"))
+        assert rgbd.shape == (6, 16, 3) and (rgbd[:, 8:] == dimg).all()
+    assert evaluation_path(data, None, c2ws[:1], renderer=_stub_renderer, device="cpu") == []

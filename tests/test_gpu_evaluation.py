@@ -1,6 +1,7 @@
 """evaluation() renders each test image through render_ray in chunks and
 scores it: its images equal a direct render of the same rays, and its PSNR
-and SSIM equal the metrics applied to those images."""
+and SSIM equal the metrics applied to those images.  evaluation_path's
+frames equal direct renders of the rays of its cameras."""
 import sys
 
 import pytest
@@ -30,3 +31,28 @@ def test_evaluation_matches_direct_render():
         # float64 convolutions on the GPU and the CPU differ in the last digits
         assert abs(out["ssim"][i] - ssim(want, gt[i].reshape(H, W, 3))) < 1e-12
     assert abs(out["psnr_mean"] - sum(out["psnr"]) / n) < 1e-12
+
+
+def test_evaluation_path_matches_direct_render():
+    print("[case] evaluation_path", file=sys.stderr, flush=True)
+    import numpy as np
+
+    from ffields.evaluation import evaluation_path
+    from ffields.rays import get_ray_directions, get_rays
+
+    class Path:
+        img_wh = (10, 7)
+        directions = get_ray_directions(7, 10, (8.0, 8.0))
+        near_far = [2.0, 6.0]
+
+    m = make_model(seed=8)
+    # cameras at distance 4 on the -y and +x axes, looking at the origin
+    c2ws = [torch.tensor([[1.0, 0, 0, 0], [0, 0, 1, -4], [0, -1, 0, 0]]),
+            torch.tensor([[0.0, 0, -1, 4], [0, 1, 0, 0], [1, 0, 0, 0]])]
+    out = evaluation_path(Path, m, c2ws, chunk=33, white_bg=True, return_maps=True)
+    for k, c2w in enumerate(c2ws):
+        o, d = get_rays(Path.directions, c2w)
+        with torch.no_grad():
+            rgb, _, _ = m(torch.cat([o, d], 1).cuda(), white_bg=True, is_train=False)
+        want = (rgb.clamp(0, 1).reshape(7, 10, 3).cpu().numpy() * 255).astype(np.uint8)
+        assert (out["rgb_maps"][k] == want).all()
