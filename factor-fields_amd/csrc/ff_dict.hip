@@ -61,8 +61,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 // K slot (ks, g, j) of a 32-wide step -> index.
 __device__ __forceinline__ int dperm(int ks, int g, int j) { return 16 * (2 * ks + (j >> 2)) + 4 * g + (j & 3); }
 
+// Weight loads are unconditional, at indices clamped into the matrix, with the
+// zero padding selected afterwards, so that they are not serialised behind
+// per-lane branches.
 __device__ __forceinline__ float w0a(const ff_mlp_desc& m, int h, int k) {
-  return k < kDC ? m.w0[h * kDC + k] : (k == kDC ? m.b0[h] : 0.f);
+  const float w = m.w0[h * kDC + (k < kDC ? k : kDC - 1)];
+  const float b = m.b0[h];
+  return k < kDC ? w : (k == kDC ? b : 0.f);
 }
 
 // Block-shared A operands (BF16): layer 1 [hidden tile][K step][lane] and the
@@ -257,7 +262,10 @@ __device__ __forceinline__ void load_a2(const ff_mlp_desc& m, dbf8 (&a2)[2]) {
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) a2[ks][j] = (__bf16)(r < m.out_dim ? m.w1[r * kDH + dperm(ks, g, j)] : 0.f);
+    for (int j = 0; j < 8; ++j) {
+      const float v = m.w1[(r < m.out_dim ? r : m.out_dim - 1) * kDH + dperm(ks, g, j)];
+      a2[ks][j] = (__bf16)(r < m.out_dim ? v : 0.f);
+    }
 }
 
 template <bool BF>
@@ -315,7 +323,8 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int o = 8 * g + j;
-      a3[mt][j] = (__bf16)(o < O ? m.w1[o * kDH + 16 * mt + r] : 0.f);
+      const float v = m.w1[(o < O ? o : O - 1) * kDH + 16 * mt + r];
+      a3[mt][j] = (__bf16)(o < O ? v : 0.f);
     }
   df16 acc0[5][2], acc1[2];
 #pragma unroll

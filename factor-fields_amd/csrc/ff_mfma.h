@@ -73,9 +73,15 @@ __device__ __forceinline__ int mf_hid(int ks, int q, int j) {
   return 32 * (ks >> 1) + 8 * ((ks & 1) * 2 + (j >> 2)) + 4 * q + (j & 3);
 }
 
-// W0 with b0 as column 18 and zero columns 19..31.
+// W0 with b0 as column 18 and zero columns 19..31.  The weight loads in this
+// file are unconditional, at indices clamped into the matrix, and the zero
+// padding is selected afterwards: a load under a per-lane condition becomes a
+// branch that waits for its load before the next one issues, which serialised
+// the whole operand set (about a hundred memory latencies per wave).
 __device__ __forceinline__ float w0_aug(const ff_mlp_desc& m, int h, int k) {
-  return k < kMfIn ? m.w0[h * kMfIn + k] : (k == kMfIn ? m.b0[h] : 0.f);
+  const float w = m.w0[h * kMfIn + (k < kMfIn ? k : kMfIn - 1)];
+  const float b = m.b0[h];
+  return k < kMfIn ? w : (k == kMfIn ? b : 0.f);
 }
 
 // Decoder weights in operand form (BF16, rounded from the FP32 masters).
@@ -93,7 +99,10 @@ struct MfmaDecoder {
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) w1a[ks][j] = (__bf16)(r < kMfO ? m.w1[r * kMfH + mf_hid(ks, q, j)] : 0.f);
+      for (int j = 0; j < 8; ++j) {
+        const float v = m.w1[(r < kMfO ? r : kMfO - 1) * kMfH + mf_hid(ks, q, j)];
+        w1a[ks][j] = (__bf16)(r < kMfO ? v : 0.f);
+      }
   }
 };
 
@@ -111,7 +120,8 @@ struct MfmaDecoderBwd {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           const int o = ks * 16 + 8 * q + j;
-          w1t[mt][ks][j] = (__bf16)(o < kMfO ? m.w1[o * kMfH + mt * 32 + r] : 0.f);
+          const float v = m.w1[(o < kMfO ? o : kMfO - 1) * kMfH + mt * 32 + r];
+          w1t[mt][ks][j] = (__bf16)(o < kMfO ? v : 0.f);
         }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
