@@ -486,6 +486,18 @@ def run_image(args):
     }
 
 
+def _cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the baseline's record."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline_sdf(args):
     """The oracle (same ATen ops as the reference: grid_sample per level,
     autograd, torch.optim.Adam) on the host cores, bounded sample."""
@@ -528,7 +540,8 @@ def cpu_baseline_sdf(args):
     med = times[len(times) // 2]
     return {"value": n / med, "unit": "field pts/s", "cores": cores, "kind": "port",
             "sample": f"{len(times)} timed steps (median) of {n} points of the same SDF workload, "
-                      f"oracle (torch CPU, grid_sample per level + autograd + Adam)"}
+                      f"oracle (torch CPU, grid_sample per level + autograd + Adam), {cores} threads on "
+                      f"{_cpu_model()}"}
 
 
 def cpu_baseline_nerf(args):
@@ -582,7 +595,8 @@ def cpu_baseline_nerf(args):
     med = times[len(times) // 2]
     return {"value": n / med, "unit": "rays/s", "cores": cores, "kind": "port",
             "sample": f"{len(times)} timed steps (median) of {n} rays of the same scene and model, plain form "
-                      f"without occupancy grid ({n_samples} samples/ray), oracle (torch CPU, autograd + Adam)"}
+                      f"without occupancy grid ({n_samples} samples/ray), oracle (torch CPU, autograd + Adam), "
+                      f"{cores} threads on {_cpu_model()}"}
 
 
 def main():
