@@ -147,9 +147,10 @@ def evaluation_path(test_dataset, factor_fields, c2ws, renderer=render_ray, save
         os.makedirs(os.path.join(savePath, "rgbd"), exist_ok=True)
     near_far = test_dataset.near_far
     PSNRs, rgb_maps, depth_maps = [], [], []
+    directions = torch.as_tensor(test_dataset.directions, dtype=torch.float32)
     for idx, c2w in enumerate(c2ws):
-        c2w = torch.as_tensor(np.asarray(c2w), dtype=torch.float32)
-        rays_o, rays_d = get_rays(torch.as_tensor(test_dataset.directions, dtype=torch.float32), c2w)
+        c2w = torch.as_tensor(c2w, dtype=torch.float32, device=directions.device)
+        rays_o, rays_d = get_rays(directions, c2w)
         rays = torch.cat([rays_o, rays_d], 1)
         rgb_map, depth_map = renderer(rays, factor_fields, chunk=chunk, N_samples=N_samples, ndc_ray=ndc_ray,
                                       white_bg=white_bg, device=device)

@@ -129,3 +129,7 @@ def test_evaluation_path_frames(tmp_path):
         rgbd = np.asarray(Image.open(tmp_path / "rgbd" / f"p_{k:03d}.png"))
         assert rgbd.shape == (6, 16, 3) and (rgbd[:, 8:] == dimg).all()
     assert evaluation_path(data, None, c2ws[:1], renderer=_stub_renderer, device="cpu") == []
+    # poses as numpy arrays give the same frames
+    out2 = evaluation_path(data, None, [c.numpy() for c in c2ws], renderer=_stub_renderer, device="cpu",
+                           return_maps=True)
+    assert (out2["rgb_maps"] == out["rgb_maps"]).all()
