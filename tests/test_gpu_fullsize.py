@@ -1,0 +1,156 @@
+"""Full-size cases of the benchmarked workloads, through properties that do
+not depend on the batch size.
+
+* Every ray and every point is computed independently, so a subset evaluated
+  on its own equals the same entries of the full batch bit for bit (packed
+  sample offsets, capacities and tile assignment change with the batch; the
+  values must not).  The subset is also checked against the oracle.
+* The dictionary's table gradients do not depend on the order of its 2^22
+  pixels, up to FP32 summation order (atomics and per-wave partial sums):
+  the bench's image-and-cell order against draw order.  The same holds for
+  the SDF regression's gradients.
+
+Shapes: NeRF 4,096 rays of the 800x800 synthetic cameras, 3-D grid basis
+[4,4,4,2,2,2]@[32..128] (reference parameter derivation), 32^3x18
+coefficient grid, 128^3 occupancy grid, uniform sampler at the reference's
+step; SDF 2^20 points, VM coefficient 3x128^2 + 3x128; dictionary 1,024
+images of 256x256, basis [32,32,32,16,16,16], 1,024 x 32x32x144 BF16
+coefficient grid, decoder 144-64-3, 2^22 pixels."""
+import sys
+
+import pytest
+import torch
+
+from ffields import FactorFields
+from ffields import workloads as W
+from ffields.render import AlphaGridMask
+from helpers import assert_close, oracle_of, randomize
+from test_gpu_dict import emulate
+from test_gpu_render import oracle_renderer
+
+pytestmark = pytest.mark.gpu
+
+
+def case(name):
+    print(f"[case] {name}", file=sys.stderr, flush=True)
+
+
+def assert_close_gpu(a, b, rtol, what, atol_frac):
+    """assert_close's bar, evaluated on the device (tables of 10^8 entries)."""
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    scale = float(b.abs().max()) or 1.0
+    err = (a - b).abs()
+    bad = err > rtol * b.abs() + atol_frac * scale
+    assert not bool(bad.any()), (what, float(err.max()), scale, int(bad.sum()))
+
+
+def nerf_model():
+    m = FactorFields(W.nerf_cfg(), "cuda")
+    randomize(m, 3, scale=0.5)
+    with torch.no_grad():
+        m.linear_mat.backbone[0].weight.mul_(20.0)    # density varies in space
+    of = oracle_of(m)
+    g = torch.Generator().manual_seed(4)
+    a = m.aabb.cpu()
+    o0 = of.field(a[0] + (a[1] - a[0]) * torch.rand((4096, 3), generator=g))[:, 0]
+    m.cfg.renderer.density_shift = float(-torch.quantile(o0, 0.6))
+    vol = (torch.rand((128, 128, 128), generator=g) < 0.6).float()
+    m.alphaMask = AlphaGridMask(torch.device("cuda"), a, vol)
+    return m
+
+
+def test_nerf_render_full_batch():
+    case("nerf full batch render: 4096 rays, 128^3 occupancy")
+    m = nerf_model()
+    dev = torch.device("cuda")
+    scene = W.NerfScene(device=dev)
+    rays = scene.random_rays(4096, torch.Generator(device=dev).manual_seed(5))
+    jit = torch.rand(4096, generator=torch.Generator(device=dev).manual_seed(6), device=dev)
+    with torch.no_grad():
+        rgb, depth, _ = m(rays, white_bg=True, is_train=False, jitter=jit)
+        idx = torch.randperm(4096, generator=torch.Generator().manual_seed(7))[:48].to(dev)
+        rgb_s, depth_s, _ = m(rays[idx].contiguous(), white_bg=True, is_train=False, jitter=jit[idx].contiguous())
+    assert torch.equal(rgb[idx], rgb_s) and torch.equal(depth[idx], depth_s)
+    orr = oracle_renderer(m)
+    ref_rgb, ref_depth, aux = orr.forward(rays[idx].cpu(), jitter=jit[idx].cpu(), white_bg=True)
+    assert float(aux["acc"].max()) > 0.5                # the rays see the field
+    assert_close(rgb_s, ref_rgb, 1e-5, "rgb_map")
+    assert_close(depth_s, ref_depth, 1e-5, "depth")
+
+
+def dict_model():
+    m = FactorFields(W.dict_cfg(1024, 256), "cuda", table_dtype="bf16")
+    randomize(m, 8)
+    assert m.fused_supported()
+    return m
+
+
+def dict_batch(n, seed):
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    img, p = W.image_set_batch(n, 1024, 256, gen, dev, order=W.cell_order(256, 32, dev))
+    return torch.stack([(p % 256).float() + 0.5, (p // 256).float() + 0.5, img.float() + 0.5], -1).contiguous()
+
+
+def test_dict_forward_full_batch():
+    case("dict full batch forward: 2^22 pixels, 1024 images")
+    m = dict_model()
+    n = 1 << 22
+    x = dict_batch(n, 9)
+    with torch.no_grad():
+        out = m.field(x)
+        idx = torch.randperm(n, generator=torch.Generator().manual_seed(10))[:2048].to(x.device)
+        sub = m.field(x[idx].contiguous())
+    assert torch.equal(out[idx], sub)
+    ref, *_ = emulate(m, oracle_of(m, bf16_tables=True).get_coding(x[idx].cpu())[0])
+    assert_close(sub, ref, 2e-3, "out", atol_frac=2e-3)
+
+
+def test_dict_gradients_full_batch_order():
+    case("dict full batch gradients: cell order vs draw order")
+    m = dict_model()
+    n = 1 << 22
+    x = dict_batch(n, 11)
+    t = torch.rand((n, 3), generator=torch.Generator(device=x.device).manual_seed(12), device=x.device)
+    perm = torch.randperm(n, generator=torch.Generator(device=x.device).manual_seed(13), device=x.device)
+    params = list(m.basises) + list(m.coeffs) + list(m.linear_mat.parameters())
+    grads, losses = [], []
+    for xi, ti in [(x, t), (x[perm].contiguous(), t[perm].contiguous())]:
+        m.zero_grad(set_to_none=True)
+        losses.append(float(m.regression_step(xi, ti, loss="mse")))
+        grads.append([p.grad.detach().clone() for p in params])
+    assert abs(losses[0] - losses[1]) <= 1e-5 * losses[0]
+    for k, (a, b) in enumerate(zip(*grads)):
+        assert float(b.abs().max()) > 0, f"param {k} has no gradient"
+        assert_close_gpu(a, b, 1e-4, f"param {k}", atol_frac=1e-5)
+
+
+def test_sdf_full_batch():
+    """SDF regression (VM coefficient 3x128^2 + 3x128, 18 channels, MLP
+    18-64-1, L1) at 2^20 points: subset invariance and the oracle for the
+    forward; gradients independent of the batch order."""
+    case("sdf full batch: 2^20 points, VM coefficient")
+    m = FactorFields(W.sdf_cfg(), "cuda")
+    randomize(m, 14, scale=0.5)
+    assert m.fused_supported()
+    dev = torch.device("cuda")
+    n = 1 << 20
+    g = torch.Generator(device=dev).manual_seed(15)
+    x = (torch.rand((n, 3), generator=g, device=dev) * 2 - 1).contiguous()
+    t = (torch.rand((n, 1), generator=g, device=dev) - 0.5).contiguous()
+    with torch.no_grad():
+        out = m.field(x)
+        idx = torch.randperm(n, generator=torch.Generator().manual_seed(16))[:4096].to(dev)
+        sub = m.field(x[idx].contiguous())
+    assert torch.equal(out[idx], sub)
+    assert_close(sub, oracle_of(m).field(x[idx].cpu()), 1e-5, "sdf out")
+    perm = torch.randperm(n, generator=torch.Generator(device=dev).manual_seed(17), device=dev)
+    params = list(m.basises) + list(m.coeffs) + list(m.linear_mat.parameters())
+    grads = []
+    for xi, ti in [(x, t), (x[perm].contiguous(), t[perm].contiguous())]:
+        m.zero_grad(set_to_none=True)
+        m.regression_step(xi, ti, loss="l1")
+        grads.append([p.grad.detach().clone() for p in params])
+    for k, (a, b) in enumerate(zip(*grads)):
+        assert float(b.abs().max()) > 0, f"param {k} has no gradient"
+        assert_close_gpu(a, b, 1e-4, f"param {k}", atol_frac=1e-5)
