@@ -44,8 +44,8 @@ def assert_close_gpu(a, b, rtol, what, atol_frac):
     assert not bool(bad.any()), (what, float(err.max()), scale, int(bad.sum()))
 
 
-def nerf_model():
-    m = FactorFields(W.nerf_cfg(), "cuda")
+def nerf_model(dtype="fp32"):
+    m = FactorFields(W.nerf_cfg(), "cuda", table_dtype=dtype)
     randomize(m, 3, scale=0.5)
     with torch.no_grad():
         m.linear_mat.backbone[0].weight.mul_(20.0)    # density varies in space
@@ -59,9 +59,13 @@ def nerf_model():
     return m
 
 
-def test_nerf_render_full_batch():
-    case("nerf full batch render: 4096 rays, 128^3 occupancy")
-    m = nerf_model()
+@pytest.mark.parametrize("dtype", ["fp32", "bf16"])
+def test_nerf_render_full_batch(dtype):
+    """FP32 form: the subset within 1e-5 of the FP32 oracle.  BF16 tables and
+    matrix-core decoder: mean absolute rgb error against the same FP32
+    oracle below 1e-2, the bar of the small-size render test."""
+    case(f"nerf full batch render ({dtype}): 4096 rays, 128^3 occupancy")
+    m = nerf_model(dtype)
     dev = torch.device("cuda")
     scene = W.NerfScene(device=dev)
     rays = scene.random_rays(4096, torch.Generator(device=dev).manual_seed(5))
@@ -74,8 +78,14 @@ def test_nerf_render_full_batch():
     orr = oracle_renderer(m)
     ref_rgb, ref_depth, aux = orr.forward(rays[idx].cpu(), jitter=jit[idx].cpu(), white_bg=True)
     assert float(aux["acc"].max()) > 0.5                # the rays see the field
-    assert_close(rgb_s, ref_rgb, 1e-5, "rgb_map")
-    assert_close(depth_s, ref_depth, 1e-5, "depth")
+    if dtype == "fp32":
+        assert_close(rgb_s, ref_rgb, 1e-5, "rgb_map")
+        assert_close(depth_s, ref_depth, 1e-5, "depth")
+    else:
+        err = (rgb_s.double().cpu() - ref_rgb.double()).abs()
+        print(f"[bf16 full render] max abs error {err.max().item():.3e}, mean {err.mean().item():.3e}",
+              file=sys.stderr, flush=True)
+        assert err.mean().item() < 1e-2
 
 
 def dict_model():
