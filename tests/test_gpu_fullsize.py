@@ -8,7 +8,9 @@ not depend on the batch size.
 * The dictionary's table gradients do not depend on the order of its 2^22
   pixels, up to FP32 summation order (atomics and per-wave partial sums):
   the bench's image-and-cell order against draw order.  The same holds for
-  the SDF regression's gradients.
+  the SDF regression's gradients and for the proposal + BF16 NeRF training
+  pass (its random draws injected), whose training render permutes bit for
+  bit with the rays.
 
 Shapes: NeRF 4,096 rays of the 800x800 synthetic cameras, 3-D grid basis
 [4,4,4,2,2,2]@[32..128] (reference parameter derivation), 32^3x18
@@ -164,3 +166,44 @@ def test_sdf_full_batch():
     for k, (a, b) in enumerate(zip(*grads)):
         assert float(b.abs().max()) > 0, f"param {k} has no gradient"
         assert_close_gpu(a, b, 1e-4, f"param {k}", atol_frac=1e-5)
+
+
+def test_proposal_train_step_full_batch():
+    """The benchmarked training pass (BF16 tables, learned proposal with a
+    64^3x8 grid placing 48 samples on each of 4,096 rays) with its random
+    draws injected: permuting the rays, with their draws and targets,
+    permutes the training render bit for bit and leaves every gradient (field
+    tables, decoder, proposal grid and MLP) unchanged up to FP32 summation
+    order, and the loss with it."""
+    case("proposal training pass, full batch: 4096 rays, ray order")
+    m = nerf_model("bf16")
+    m.enable_proposal(reso=64, n_fine=48, table_dtype="bf16")
+    dev = torch.device("cuda")
+    B = 4096
+    g = torch.Generator(device=dev).manual_seed(18)
+    rays = W.NerfScene(device=dev).random_rays(B, g)
+    target = torch.rand((B, 3), generator=g, device=dev)
+    prop_u = torch.rand((B, 64), generator=g, device=dev)
+    fine_u = torch.rand((B, 48), generator=g, device=dev)
+    perm = torch.randperm(B, generator=g, device=dev)
+    params = list(m.parameters())
+    assert any(p is m.proposal.grid for p in params)
+    runs = []
+    for order in (None, perm):
+        def sel(t):
+            return t if order is None else t[order].contiguous()
+        m.zero_grad(set_to_none=True)
+        loss = m.render_train_step_proposal(sel(rays), sel(target), white_bg=True, prop_u=sel(prop_u),
+                                            fine_u=sel(fine_u))
+        runs.append((float(loss), m.last_rgb_map().clone(),
+                     [None if p.grad is None else p.grad.detach().clone() for p in params]))
+    (l0, r0, g0), (l1, r1, g1) = runs
+    assert torch.equal(r0[perm], r1)
+    assert abs(l0 - l1) <= 1e-5 * abs(l0)
+    n_grads = 0
+    for k, (a, b) in enumerate(zip(g0, g1)):
+        assert (a is None) == (b is None), f"param {k}"
+        if a is not None:
+            n_grads += 1
+            assert_close_gpu(a, b, 1e-4, f"param {k}", atol_frac=1e-5)
+    assert n_grads >= 6 + 1 + 3 + 4     # basis levels, coefficient, decoder, proposal grid + MLP
