@@ -8,14 +8,15 @@ not depend on the batch size.
 * The dictionary's table gradients do not depend on the order of its 2^22
   pixels, up to FP32 summation order (atomics and per-wave partial sums):
   the bench's image-and-cell order against draw order.  The same holds for
-  the SDF regression's gradients and for the proposal + BF16 NeRF training
-  pass (its random draws injected), whose training render permutes bit for
-  bit with the rays.
+  the SDF and image regressions' gradients and for the proposal + BF16 NeRF
+  training pass (its random draws injected), whose training render permutes
+  bit for bit with the rays.
 
 Shapes: NeRF 4,096 rays of the 800x800 synthetic cameras, 3-D grid basis
 [4,4,4,2,2,2]@[32..128] (reference parameter derivation), 32^3x18
 coefficient grid, 128^3 occupancy grid, uniform sampler at the reference's
-step; SDF 2^20 points, VM coefficient 3x128^2 + 3x128; dictionary 1,024
+step; SDF 2^20 points, VM coefficient 3x128^2 + 3x128; image 65,536 pixels
+of a 512x512 image, 64x64x18 coefficient grid; dictionary 1,024
 images of 256x256, basis [32,32,32,16,16,16], 1,024 x 32x32x144 BF16
 coefficient grid, decoder 144-64-3, 2^22 pixels."""
 import sys
@@ -207,3 +208,36 @@ def test_proposal_train_step_full_batch():
             n_grads += 1
             assert_close_gpu(a, b, 1e-4, f"param {k}", atol_frac=1e-5)
     assert n_grads >= 6 + 1 + 3 + 4     # basis levels, coefficient, decoder, proposal grid + MLP
+
+
+def test_image_full_batch():
+    """2-D image regression (512x512 image, grid basis [4,4,4,2,2,2]@[32..128],
+    64x64x18 coefficient grid, MLP 18-64-3, MSE) at its 65,536-pixel batch:
+    subset invariance and the oracle for the forward; gradients independent
+    of the batch order."""
+    case("image full batch: 65536 pixels")
+    m = FactorFields(W.image_cfg(512), "cuda")
+    randomize(m, 19, scale=0.5)
+    assert m.fused_supported()
+    dev = torch.device("cuda")
+    n = 65536
+    g = torch.Generator(device=dev).manual_seed(20)
+    coords = W.pixel_coords(512, 512, device=dev)
+    x = coords[torch.randint(0, 512 * 512, (n,), generator=g, device=dev)].contiguous()
+    t = torch.rand((n, 3), generator=g, device=dev)
+    with torch.no_grad():
+        out = m.field(x)
+        idx = torch.randperm(n, generator=torch.Generator().manual_seed(21))[:4096].to(dev)
+        sub = m.field(x[idx].contiguous())
+    assert torch.equal(out[idx], sub)
+    assert_close(sub, oracle_of(m).field(x[idx].cpu()), 1e-5, "image out")
+    perm = torch.randperm(n, generator=g, device=dev)
+    params = list(m.basises) + list(m.coeffs) + list(m.linear_mat.parameters())
+    grads = []
+    for xi, ti in [(x, t), (x[perm].contiguous(), t[perm].contiguous())]:
+        m.zero_grad(set_to_none=True)
+        m.regression_step(xi, ti, loss="mse")
+        grads.append([p.grad.detach().clone() for p in params])
+    for k, (a, b) in enumerate(zip(*grads)):
+        assert float(b.abs().max()) > 0, f"param {k} has no gradient"
+        assert_close_gpu(a, b, 1e-4, f"param {k}", atol_frac=1e-5)
