@@ -241,3 +241,26 @@ def test_image_full_batch():
     for k, (a, b) in enumerate(zip(*grads)):
         assert float(b.abs().max()) > 0, f"param {k} has no gradient"
         assert_close_gpu(a, b, 1e-4, f"param {k}", atol_frac=1e-5)
+
+
+def test_dense_alpha_full_grid():
+    """getDenseAlpha (models/FactorFields.py:729-755) on the full 128^3 grid
+    of the NeRF model with its 128^3 occupancy mask, one unjittered pass:
+    4,096 grid points drawn at random match the oracle's alpha there."""
+    from oracle import alpha_grid_sample
+    case("dense alpha, 128^3 grid")
+    m = nerf_model()
+    R = [128, 128, 128]
+    alpha, dense = m.getDenseAlpha(R, times=1)
+    assert alpha.shape == (128, 128, 128)
+    idx = torch.randperm(128 ** 3, generator=torch.Generator().manual_seed(22))[:4096]
+    x = dense.reshape(-1, 3).cpu()[idx]
+    of = oracle_of(m)
+    sig = of.basis2density(of.field(x)[:, 0])
+    am = m.alphaMask
+    keep = alpha_grid_sample(am.alpha_volume.cpu()[0, 0], am.aabb.cpu(), x) > 0
+    units = (m.aabb[1] - m.aabb[0]).cpu() / (torch.tensor(R) - 1)
+    length = float(units.mean() * 25.0)
+    ref = torch.where(keep, 1 - torch.exp(-sig * length), torch.zeros_like(sig))
+    assert bool(keep.any()) and bool((~keep).any())
+    assert_close(alpha.reshape(-1)[idx.to(alpha.device)], ref, 1e-5, "alpha")
