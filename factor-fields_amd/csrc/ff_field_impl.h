@@ -73,18 +73,48 @@ __device__ __forceinline__ void basis_level_fwd(const ff_field_desc& fd, int l, 
     const int xb = min(max(i0[0], 0), R - 2);
     const bool sel = i0[0] == xb;  // x0 is the pair's first texel (else its second)
     const bool okx0 = i0[0] >= 0 && i0[0] < R, okx1 = i0[0] + 1 >= 0 && i0[0] + 1 < R;
+    // BF16 tables: all rows' pair loads are issued before any is used (the
+    // scheduler otherwise waits for each one: 45.2 -> 42.6 us for the render
+    // forward's batch).  FP32 tables keep one loop that loads each pair just
+    // before its use, which measured slightly faster for the FP32 kernels.
+    constexpr int NP = 1 << (D - 1);
+    float v2s[BF ? NP : 1][2 * dl];
+    bool oks[BF ? NP : 1];
+    if constexpr (BF) {
 #pragma unroll
-    for (int pr = 0; pr < (1 << (D - 1)); ++pr) {
-      int row = 0;
-      bool okr = true;
+      for (int pr = 0; pr < NP; ++pr) {
+        int row = 0;
+        bool okr = true;
 #pragma unroll
-      for (int a = D - 1; a >= 1; --a) {
-        const int ii = i0[a] + ((pr >> (a - 1)) & 1);
-        okr = okr && ii >= 0 && ii < R;
-        row = row * R + min(max(ii, 0), R - 1);
+        for (int a = D - 1; a >= 1; --a) {
+          const int ii = i0[a] + ((pr >> (a - 1)) & 1);
+          okr = okr && ii >= 0 && ii < R;
+          row = row * R + min(max(ii, 0), R - 1);
+        }
+        oks[pr] = okr;
+        load_ch<2 * dl, BF>(fd.basis[l], ((int64_t)row * R + xb) * dl, v2s[pr]);
       }
-      float v2[2 * dl];
-      load_ch<2 * dl, BF>(fd.basis[l], ((int64_t)row * R + xb) * dl, v2);
+    }
+#pragma unroll
+    for (int pr = 0; pr < NP; ++pr) {
+      bool okr;
+      const float* v2;
+      float v2l[2 * dl];
+      if constexpr (BF) {
+        okr = oks[pr];
+        v2 = v2s[pr];
+      } else {
+        int row = 0;
+        okr = true;
+#pragma unroll
+        for (int a = D - 1; a >= 1; --a) {
+          const int ii = i0[a] + ((pr >> (a - 1)) & 1);
+          okr = okr && ii >= 0 && ii < R;
+          row = row * R + min(max(ii, 0), R - 1);
+        }
+        load_ch<2 * dl, BF>(fd.basis[l], ((int64_t)row * R + xb) * dl, v2l);
+        v2 = v2l;
+      }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int k = 2 * pr + h;
