@@ -141,8 +141,12 @@ class FactorFields(RenderMixin, nn.Module):
 
     # ------------------------------------------------ kernel description
     def _tables(self):
-        b = list(self.basises) if self.basis_type != "none" else []
-        c = list(self.coeffs) if self.coeff_type != "none" else []
+        # The ParameterLists' own dicts hold the tables in index order;
+        # iterating a ParameterList goes through its string-keyed __getitem__
+        # (about 17 us per call here), and every training step's host code
+        # asks for the tables many times.
+        b = list(self.basises._parameters.values()) if self.basis_type != "none" else []
+        c = list(self.coeffs._parameters.values()) if self.coeff_type != "none" else []
         return b, c
 
     def refresh_shadows(self):
