@@ -150,7 +150,8 @@ class FactorFields(RenderMixin, nn.Module):
         return b, c
 
     def refresh_shadows(self):
-        """(Re)build the BF16 copies the kernels gather from (BF16 mode)."""
+        """(Re)build the BF16 copies the kernels gather from (BF16 mode), and
+        remember which master tensors, at which versions, they were cast from."""
         b, c = self._tables()
         self._shadows = []
         for t in b + c:
@@ -160,6 +161,21 @@ class FactorFields(RenderMixin, nn.Module):
                             memory_format=torch.channels_last if t.dim() == 4 else torch.channels_last_3d)
             L.check(L.lib().ff_cast_bf16(L.ptr(t), L.ptr(s), t.numel(), L.stream_of(t)), "ff_cast_bf16")
             self._shadows.append(s)
+        self._shadow_src = [(t, t._version) for t in b + c]
+
+    def _shadows_current(self, tabs):
+        """True while every FP32 master is the tensor, at the version, its BF16
+        copy was cast from.  Any in-place write through torch (torch.optim.Adam,
+        load_state_dict, copy_) bumps the version.  FusedAdam writes the masters
+        and their copies together through raw pointers, which leaves the
+        version alone and the copies current."""
+        src = getattr(self, "_shadow_src", None)
+        if src is None or len(src) != len(tabs):
+            return False
+        for t, (s, v) in zip(tabs, src):
+            if t is not s or t._version != v:
+                return False
+        return True
 
     def shadow_of(self, p):
         b, c = self._tables()
@@ -205,6 +221,8 @@ class FactorFields(RenderMixin, nn.Module):
     def _read_tables(self):
         b, c = self._tables()
         if self.table_dtype == "bf16":
+            if not self._shadows_current(b + c):
+                self.refresh_shadows()
             return list(self._shadows)
         for t in b + c:
             if not is_texel_major(t):

@@ -44,6 +44,7 @@ class ProposalNet(nn.Module):
         s = torch.empty(t.shape, dtype=torch.bfloat16, device=t.device, memory_format=torch.channels_last_3d)
         L.check(L.lib().ff_cast_bf16(L.ptr(t), L.ptr(s), t.numel(), L.stream_of(t)), "ff_cast_bf16")
         self._shadow = s
+        self._shadow_src = (t, t._version)
 
     def shadow_of(self, p):
         return self._shadow if (p is self.grid and self.table_dtype == "bf16") else None
@@ -52,6 +53,13 @@ class ProposalNet(nn.Module):
         d = L.ProposalDesc()
         d.reso[0] = d.reso[1] = d.reso[2] = self.reso
         d.table_dtype = L.BF16 if self.table_dtype == "bf16" else L.F32
+        if self.table_dtype == "bf16":
+            # the grid was written through torch since the copy was cast (its
+            # version moved, or it was replaced): cast again; FusedAdam keeps
+            # both current without touching the version
+            src, ver = self._shadow_src
+            if src is not self.grid or src._version != ver:
+                self.refresh_shadow()
         d.grid = (self._shadow if self.table_dtype == "bf16" else self.grid).data_ptr()
         d.w0 = self.mlp[0].weight.data_ptr()
         d.b0 = self.mlp[0].bias.data_ptr()
