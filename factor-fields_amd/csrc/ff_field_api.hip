@@ -252,6 +252,8 @@ const char* ff_status_str(int status) {
     case FF_ERR_UNSUPPORTED: return "configuration has no kernel instantiation";
     case FF_ERR_WORKSPACE: return "workspace missing or too small";
     case FF_ERR_LAUNCH: return "kernel launch failed";
+    case FF_ERR_SCRATCH_LIMIT:
+      return "request exceeds the static scratch of the allocation-free entry point";
     default: return "unknown status";
   }
 }

@@ -8,6 +8,8 @@
 // grid is persistent and every wave strides over 64-sample tiles.  With BF16
 // tables the decoder runs on the matrix cores (BF16 operands, FP32
 // accumulation, ff_mfma.h); the FP32 form keeps the FP32 vector decoder.
+#include <mutex>
+
 #include "ff_field_impl.h"
 #include "ff_mfma.h"
 
@@ -419,9 +421,14 @@ size_t ff_render_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_m
   return ((size_t)rf_blocks(cap, 64, kBwdBlocksPerCu) + kReduceSplit) * kNerfPartial * 4;
 }
 
-int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
-                        const float* rays, int64_t n_rays, const ff_samples* s, const float* d_sample,
-                        const ff_grads* grads, void* ws, size_t ws_bytes, void* stream) {
+}  // extern "C"
+
+namespace ff {
+namespace {
+
+int render_field_bwd_impl(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
+                          const float* rays, int64_t n_rays, const ff_samples* s, const float* d_sample,
+                          const ff_grads* grads, void* ws, size_t ws_bytes, void* stream) {
   if (!desc || !mlp || !head || !s || !s->ray_off || !s->xyz || !s->ray || !rays || !d_sample || !grads)
     return FF_ERR_ARG;
   if (n_rays < 0) return FF_ERR_ARG;
@@ -454,6 +461,58 @@ int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
   });
   if (r) return r;
   return launch_reduce_partials((const float*)ws, (int)blocks, 2, 18, 64, 28, g, st);
+}
+
+// Scratch of ff_render_field_bwd_noalloc: the decoder-gradient partial rows
+// (one per backward block, then the reduction's kReduceSplit rows).  Every row
+// is written before it is read, so nothing in it needs clearing between
+// calls.  The name is a host-side stand-in: host code uses only the address
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+__device__ __align__(256) float scratch[kScratchBytes / sizeof(float)];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kScratchDevices] = {};
+
+// This device's copy of `scratch`, looked up once per device (with that
+// device current) and kept.
+int scratch_addr(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) return FF_ERR_LAUNCH;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FF_ERR_LAUNCH;
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return FF_OK;
+}
+
+}  // namespace
+}  // namespace ff
+
+extern "C" {
+
+int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
+                        const float* rays, int64_t n_rays, const ff_samples* s, const float* d_sample,
+                        const ff_grads* grads, void* ws, size_t ws_bytes, void* stream) {
+  return render_field_bwd_impl(desc, mlp, head, rays, n_rays, s, d_sample, grads, ws, ws_bytes, stream);
+}
+
+size_t ff_render_field_bwd_noalloc_limit_bytes(void) { return kScratchBytes; }
+
+int ff_render_field_bwd_noalloc(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
+                                const float* rays, int64_t n_rays, const ff_samples* s, const float* d_sample,
+                                const ff_grads* grads, void* stream) {
+  if (!desc || !mlp || !s) return FF_ERR_ARG;
+  const size_t need = ff_render_field_bwd_workspace_bytes(desc, mlp, s->cap);
+  if (need > kScratchBytes) return FF_ERR_SCRATCH_LIMIT;
+  void* ws = nullptr;
+  const int st = scratch_addr(&ws);
+  if (st) return st;
+  return render_field_bwd_impl(desc, mlp, head, rays, n_rays, s, d_sample, grads, ws, kScratchBytes, stream);
 }
 
 }  // extern "C"

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libffields.so")
 
 FF_MAX_LEVELS = 8
 
-FF_OK, FF_ERR_ARG, FF_ERR_UNSUPPORTED, FF_ERR_WORKSPACE, FF_ERR_LAUNCH = range(5)
+FF_OK, FF_ERR_ARG, FF_ERR_UNSUPPORTED, FF_ERR_WORKSPACE, FF_ERR_LAUNCH, FF_ERR_SCRATCH_LIMIT = range(6)
 MAP = {"sawtooth": 0, "triangle": 1, "sinc": 2, "x": 3}
 INTERP = {"bilinear": 0, "nearest": 1}
 BASIS_NONE, BASIS_GRID = 0, 1
@@ -120,6 +120,9 @@ def _declare(lib):
         "ff_render_field_bwd_workspace_bytes": (sz, [P(FieldDesc), P(MlpDesc), I64]),
         "ff_render_field_bwd": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), P(RenderHead), c_vp, I64,
                                                P(Samples), c_vp, P(Grads), c_vp, sz, c_vp]),
+        "ff_render_field_bwd_noalloc_limit_bytes": (sz, []),
+        "ff_render_field_bwd_noalloc": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), P(RenderHead), c_vp, I64,
+                                                       P(Samples), c_vp, P(Grads), c_vp]),
         "ff_render_alpha": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), P(RenderHead), P(RaySampler), c_vp,
                                            I64, c_f32, c_vp, c_vp]),
         "ff_occupancy_pool": (ctypes.c_int, [c_vp, c_i32, c_i32, c_i32, c_f32, c_f32, c_vp, c_vp]),

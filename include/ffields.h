@@ -16,7 +16,9 @@
  *     FF_ERR_LAUNCH.
  *   - The caller owns every buffer; the library keeps no allocation and no
  *     global mutable state.  Workspaces are passed in; size them with the
- *     matching *_workspace_bytes query.
+ *     matching *_workspace_bytes query.  The one exception is the
+ *     allocation-free ff_render_field_bwd_noalloc, whose scratch is a static
+ *     device array (see there).
  *   - Tables are texel-major: a grid of spatial size (D,)H,W with C channels is
  *     stored as [(D,)H,W,C] (the channels_last layout of the reference's
  *     [1,C,(D,)H,W] parameter), so one corner is one contiguous C-vector.
@@ -39,7 +41,8 @@ enum ff_status {
   FF_ERR_ARG = 1,        /* bad argument (null pointer, size, unsupported combination) */
   FF_ERR_UNSUPPORTED = 2,/* configuration without a kernel instantiation */
   FF_ERR_WORKSPACE = 3,  /* workspace missing or too small */
-  FF_ERR_LAUNCH = 4      /* hipGetLastError after the launch was not hipSuccess */
+  FF_ERR_LAUNCH = 4,     /* hipGetLastError after the launch was not hipSuccess */
+  FF_ERR_SCRATCH_LIMIT = 5 /* an allocation-free entry point would need more than its static scratch */
 };
 
 /* models/FactorFields.py:11-33 (grid_mapping) */
@@ -269,6 +272,24 @@ int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp,
                         const ff_render_head* head, const float* rays, int64_t n_rays,
                         const ff_samples* s, const float* d_sample, const ff_grads* grads, void* ws,
                         size_t ws_bytes, void* stream);
+
+/* Allocation-free variant of ff_render_field_bwd.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * decoder-gradient partial rows that ff_render_field_bwd keeps in the caller's
+ * workspace live instead in a static device array of
+ * ff_render_field_bwd_noalloc_limit_bytes() = 16 MiB, one per GPU, held from
+ * the library's first use on that GPU until the process ends; no allocation
+ * happens in the call.  Requests whose workspace
+ * (ff_render_field_bwd_workspace_bytes) exceeds it return
+ * FF_ERR_SCRATCH_LIMIT.  Same kernels and results as ff_render_field_bwd.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time on
+ * one GPU (e.g. on two streams) race; serialise them. */
+size_t ff_render_field_bwd_noalloc_limit_bytes(void);
+int ff_render_field_bwd_noalloc(const ff_field_desc* desc, const ff_mlp_desc* mlp,
+                                const ff_render_head* head, const float* rays, int64_t n_rays,
+                                const ff_samples* s, const float* d_sample, const ff_grads* grads,
+                                void* stream);
 
 /* Alpha compositing, one wavefront per ray with a multiplicative prefix scan
  * for the transmittance: raw2alpha :82-88 and the composite of :878-896
