@@ -1,7 +1,8 @@
 // BF16 matrix-core decoder for one wavefront of 64 samples (lane = sample):
-// linear_mat 18 -> 64 -> 28 (models/FactorFields.py:113-159, ReLU hidden, no
-// output bias) on v_mfma_f32_32x32x16_bf16 with FP32 accumulation, and its
-// backward (data gradient + weight gradients).
+// linear_mat 18 -> 64 -> O, O <= 32 (models/FactorFields.py:113-159, ReLU
+// hidden, no output bias; O = 28 for the SH colour head, 32 for the feature
+// head of MLPRender_Fea) on v_mfma_f32_32x32x16_bf16 with FP32 accumulation,
+// and its backward (data gradient + weight gradients).
 //
 // Orientation: samples are the N dimension.  H^T [64 x 64] = W0a [64 x 32] .
 // X^T [32 x 64], where X is the 18 features plus a constant-1 column that
@@ -29,7 +30,7 @@ namespace ff {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
 
-constexpr int kMfIn = 18, kMfH = 64, kMfO = 28;
+constexpr int kMfIn = 18, kMfH = 64;
 
 // LDS words the backward needs per wave: four bf16 matrices whose rows hold 64
 // samples (+8 padding so that the 16-byte operand reads of 16 consecutive
@@ -85,9 +86,11 @@ __device__ __forceinline__ float w0_aug(const ff_mlp_desc& m, int h, int k) {
 }
 
 // Decoder weights in operand form (BF16, rounded from the FP32 masters).
+template <int O>
 struct MfmaDecoder {
+  static_assert(O >= 1 && O <= 32, "layer 2 fills one 32-row tile");
   bf16x8_t w0a[2][2];  // A of layer 1: [hidden tile][K step]
-  bf16x8_t w1a[4];     // A of layer 2: [K step] (rows >= 28 zero)
+  bf16x8_t w1a[4];     // A of layer 2: [K step] (rows >= O zero)
   __device__ __forceinline__ void load(const ff_mlp_desc& m) {
     const int lane = threadIdx.x & 63, r = lane & 31, q = lane >> 5;
 #pragma unroll
@@ -100,14 +103,15 @@ struct MfmaDecoder {
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const float v = m.w1[(r < kMfO ? r : kMfO - 1) * kMfH + mf_hid(ks, q, j)];
-        w1a[ks][j] = (__bf16)(r < kMfO ? v : 0.f);
+        const float v = m.w1[(r < O ? r : O - 1) * kMfH + mf_hid(ks, q, j)];
+        w1a[ks][j] = (__bf16)(r < O ? v : 0.f);
       }
   }
 };
 
 // Backward-only operands: W1^T (A of dH^T = W1^T . dout^T) and W0a^T in the
 // permuted hidden order (A of dX^T = W0a^T . dH^T).
+template <int O>
 struct MfmaDecoderBwd {
   bf16x8_t w1t[2][2];
   bf16x8_t w0t[4];
@@ -120,8 +124,8 @@ struct MfmaDecoderBwd {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           const int o = ks * 16 + 8 * q + j;
-          const float v = m.w1[(o < kMfO ? o : kMfO - 1) * kMfH + mt * 32 + r];
-          w1t[mt][ks][j] = (__bf16)(o < kMfO ? v : 0.f);
+          const float v = m.w1[(o < O ? o : O - 1) * kMfH + mt * 32 + r];
+          w1t[mt][ks][j] = (__bf16)(o < O ? v : 0.f);
         }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
@@ -161,7 +165,8 @@ __device__ __forceinline__ void mf_unpack_c(f32x16_t (&c)[2], float* v) {
 // layer 2 into c2.  With HT != nullptr (training) the ReLU outputs are also
 // written to relu(H)^T in LDS and the ReLU pattern returned in msk (bit i of
 // msk[mt] = pre-activation register i of hidden tile mt > 0).
-__device__ __forceinline__ void mf_fwd_tile(const MfmaDecoder& W, const bf16x8_t (&xb)[2][2], int nt,
+template <int O>
+__device__ __forceinline__ void mf_fwd_tile(const MfmaDecoder<O>& W, const bf16x8_t (&xb)[2][2], int nt,
                                             f32x16_t& c2, __bf16* HT, uint32_t* msk) {
   const int lane = threadIdx.x & 63, r = lane & 31, q = lane >> 5;
   bf16x8_t hb[4];
@@ -194,8 +199,9 @@ __device__ __forceinline__ void mf_pack_x(const float* f, bf16x8_t (&xb)[2][2]) 
   mf_pack_b(x, xb);
 }
 
-// Forward: f [18] per lane -> out [32] per lane (28 used).
-__device__ __forceinline__ void mfma_decoder_fwd(const MfmaDecoder& W, const float* f, float* out) {
+// Forward: f [18] per lane -> out [32] per lane (O used).
+template <int O>
+__device__ __forceinline__ void mfma_decoder_fwd(const MfmaDecoder<O>& W, const float* f, float* out) {
   bf16x8_t xb[2][2];
   mf_pack_x(f, xb);
   f32x16_t c2[2];
@@ -207,13 +213,14 @@ __device__ __forceinline__ void mfma_decoder_fwd(const MfmaDecoder& W, const flo
 // Per-wave weight-gradient accumulators (C layout):
 //   gw1[nt]: dW1[o = 8(i/4) + 4q + i%4][hid = 32 nt + r]
 //   gw0[mt]: dW0a[hid = 32 mt + 8(i/4) + 4q + i%4][k = r]  (k = 18: db0)
+template <int O>
 struct MfmaDecoderAcc {
   f32x16_t gw1[2];
   f32x16_t gw0[2];
   __device__ __forceinline__ void zero() {
     gw1[0] = gw1[1] = gw0[0] = gw0[1] = zero16();
   }
-  // Partial row in the decoder-gradient layout dW0 [64][18] | db0 [64] | dW1 [28][64].
+  // Partial row in the decoder-gradient layout dW0 [64][18] | db0 [64] | dW1 [O][64].
   __device__ __forceinline__ void store(float* row) const {
     const int lane = threadIdx.x & 63, r = lane & 31, q = lane >> 5;
 #pragma unroll
@@ -221,7 +228,7 @@ struct MfmaDecoderAcc {
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int a = 8 * (i >> 2) + 4 * q + (i & 3);
-        if (a < kMfO) row[kMfH * kMfIn + kMfH + a * kMfH + 32 * t + r] = gw1[t][i];
+        if (a < O) row[kMfH * kMfIn + kMfH + a * kMfH + 32 * t + r] = gw1[t][i];
         const int h = 32 * t + a;
         if (r < kMfIn) row[h * kMfIn + r] = gw0[t][i];
         else if (r == kMfIn) row[kMfH * kMfIn + h] = gw0[t][i];
@@ -232,7 +239,8 @@ struct MfmaDecoderAcc {
 
 // Training forward: as mfma_decoder_fwd, and also leaves relu(H)^T and X^T in
 // LDS for the weight gradients and the ReLU pattern in msk[mt][nt].
-__device__ __forceinline__ void mfma_decoder_fwd_train(const MfmaDecoder& W, const float* f, float* out,
+template <int O>
+__device__ __forceinline__ void mfma_decoder_fwd_train(const MfmaDecoder<O>& W, const float* f, float* out,
                                                        uint32_t (&msk)[2][2], uint32_t* lds) {
   const int lane = threadIdx.x & 63;
   __bf16* DT = reinterpret_cast<__bf16*>(lds);
@@ -254,11 +262,12 @@ __device__ __forceinline__ void mfma_decoder_fwd_train(const MfmaDecoder& W, con
 }
 
 // Backward of mfma_decoder_fwd_train: df [18] per lane from dout [32] (0 for
-// inactive lanes and o >= 28): dH^T = W1^T . dout^T, ReLU mask, dX^T =
+// inactive lanes and o >= O): dH^T = W1^T . dout^T, ReLU mask, dX^T =
 // W0a^T . dH^T; then the wave's weight gradients from the LDS operands
 // (K = samples) are added to acc.  Ends with the LDS area free.
-__device__ __forceinline__ void mfma_decoder_bwd(const MfmaDecoderBwd& WB, const uint32_t (&msk)[2][2],
-                                                 const float* dout, float* df, MfmaDecoderAcc& acc,
+template <int O>
+__device__ __forceinline__ void mfma_decoder_bwd(const MfmaDecoderBwd<O>& WB, const uint32_t (&msk)[2][2],
+                                                 const float* dout, float* df, MfmaDecoderAcc<O>& acc,
                                                  uint32_t* lds) {
   const int lane = threadIdx.x & 63, r = lane & 31, q = lane >> 5;
   __bf16* DT = reinterpret_cast<__bf16*>(lds);

@@ -225,6 +225,49 @@ __global__ __launch_bounds__(256) void composite_fwd_kernel(ff_samples s, int64_
   }
 }
 
+// App samples (models/FactorFields.py:878-885): weight > thres, with the
+// arithmetic of composite_fwd_kernel so both kernels agree on every sample.
+// pass 0: count per ray into off[b + 1]; pass 1: write the sample indices in
+// ray-major order, and each live sample's position (or -1) into app_pos.
+__global__ __launch_bounds__(256) void select_app_kernel(ff_samples s, int64_t B, const float* __restrict__ sigma,
+                                                         float thres, int pass, int32_t* __restrict__ off,
+                                                         int32_t* __restrict__ rows,
+                                                         int32_t* __restrict__ app_pos) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (b >= B) return;
+  const int64_t s0 = s.ray_off[b];
+  const int cnt = s.ray_off[b + 1] - (int)s0;
+  const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int base = pass == 1 ? off[b] : 0;
+  int napp = 0;
+  float carry = 1.f;
+  for (int r0 = 0; r0 < cnt; r0 += 64) {
+    const int j = r0 + lane;
+    const bool act = j < cnt;
+    const int64_t k = s0 + j;
+    const float sg = act ? sigma[k] : 0.f;
+    const float dd = act ? s.dist[k] : 0.f;
+    const float alpha = 1.f - expf(-sg * dd);
+    const float f = act ? (1.f - alpha) + 1e-10f : 1.f;
+    const float incl = wave_prod_scan(f, lane);
+    float excl = shfl_up_f(incl, 1);
+    if (lane == 0) excl = 1.f;
+    const float T = carry * excl;
+    const float w = act ? alpha * T : 0.f;
+    const bool app = act && w > thres;
+    const uint64_t mask = __ballot(app);
+    if (pass == 1 && act) {
+      const int pos = base + napp + __popcll(mask & lower);
+      if (app) rows[pos] = (int32_t)k;
+      if (app_pos) app_pos[k] = app ? pos : -1;
+    }
+    napp += __popcll(mask);
+    carry = carry * __shfl(incl, 63, 64);
+  }
+  if (pass == 0 && lane == 0) off[b + 1] = napp;
+}
+
 constexpr int kMaxRounds = 32;  // n_samples <= 2048
 
 __global__ __launch_bounds__(256) void composite_bwd_kernel(
@@ -421,6 +464,21 @@ int ff_composite_bwd(const ff_samples* s, int64_t n_rays, const float* sigma, co
   hipLaunchKernelGGL(composite_bwd_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, *s, n_rays,
                      sigma, rgb, white_bg, weight_thres, rgb_raw, d_rgb_map, target, loss_scale,
                      loss_sum, d_sample);
+  return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
+}
+
+int ff_select_app(const ff_samples* s, int64_t n_rays, const float* sigma, float weight_thres, int32_t* app_off,
+                  int32_t* rows, int32_t* app_pos, void* stream) {
+  if (!s || !s->ray_off || !s->dist || n_rays < 0 || !app_off) return FF_ERR_ARG;
+  if (n_rays > 0 && (!sigma || !rows)) return FF_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (n_rays == 0) return launch_zero_i32(app_off, 1, st);
+  const unsigned blocks = (unsigned)((n_rays * 64 + 255) / 256);
+  hipLaunchKernelGGL(select_app_kernel, dim3(blocks), dim3(256), 0, st, *s, n_rays, sigma, weight_thres, 0, app_off,
+                     rows, app_pos);
+  hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(1024), 0, st, app_off, n_rays);
+  hipLaunchKernelGGL(select_app_kernel, dim3(blocks), dim3(256), 0, st, *s, n_rays, sigma, weight_thres, 1, app_off,
+                     rows, app_pos);
   return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
 }
 

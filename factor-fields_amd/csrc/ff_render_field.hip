@@ -1,9 +1,14 @@
 // NeRF field over a packed sample list: get_coding + linear_mat + density
-// activation + degree-2 SH colour head in one kernel, and its fused backward.
+// activation + colour head in one kernel, and its fused backward.
 //
 // Reference: FactorFields.forward models/FactorFields.py:874-885 (get_coding,
-// linear_mat, basis2density :639-643) with the colour head
-// sigmoid(eval_sh(2, feat[1:28].view(3, 9), viewdir)) (models/sh.py:34-64).
+// linear_mat, basis2density :639-643).  Two heads (template HEAD):
+//   kHeadSH: sigmoid(eval_sh(2, feat[1:28].view(3, 9), viewdir))
+//            (models/sh.py:34-64), decoder 18 -> 64 -> 28;
+//   kHeadFeat: the features feat[1:32] of the reference's own colour module
+//            (MLPRender_Fea, evaluated afterwards on the app samples by
+//            ff_color_mlp_fwd), decoder 18 -> 64 -> 32; the backward takes
+//            their gradients from ff_color_mlp_bwd through app_pos.
 // The sample count is read on the device (ff_samples.ray_off[n_rays]); the
 // grid is persistent and every wave strides over 64-sample tiles.  With BF16
 // tables the decoder runs on the matrix cores (BF16 operands, FP32
@@ -63,11 +68,24 @@ __device__ __forceinline__ float softplus_grad(float x) {
   return z / (z + 1.f);
 }
 
-// d out for one sample: softplus/ReLU density and sigmoid(SH) colour heads.
+constexpr int kHeadSH = 0, kHeadFeat = 1;
+template <int HEAD> constexpr int head_out() { return HEAD == kHeadSH ? 28 : 32; }
+
+// d out for one sample: softplus/ReLU density and sigmoid(SH) colour heads;
+// with the feature head, the colour part is row app_pos[i] of d_feat (zero
+// for samples the colour module did not see).
+template <int HEAD>
 __device__ __forceinline__ void nerf_dout(const ff_render_head& hd, const float* o, float4 gs, float dx, float dy,
-                                          float dz, float* dout) {
+                                          float dz, int64_t i, float* dout) {
   const float x0 = o[0] + hd.density_shift;
   dout[0] = gs.x * (hd.density_act == 0 ? softplus_grad(x0) : (x0 > 0.f ? 1.f : 0.f));
+  if constexpr (HEAD == kHeadFeat) {
+    const int32_t pos = hd.app_pos[i];
+    const float* g = hd.d_feat + (int64_t)(pos >= 0 ? pos : 0) * hd.feat_stride;
+#pragma unroll
+    for (int c = 0; c < 31; ++c) dout[1 + c] = pos >= 0 ? g[c] : 0.f;
+    return;
+  }
   float Y[9];
   sh2_basis(dx, dy, dz, Y);
   const float gc[3] = {gs.y, gs.z, gs.w};
@@ -109,7 +127,23 @@ __device__ __forceinline__ void coding_of(const ff_field_desc& fd, const ff_samp
   }
 }
 
-template <int CK, bool BF>
+// The head's outputs of sample i: SH colour into rgb, or the feature row.
+template <int HEAD>
+__device__ __forceinline__ void head_out_store(const ff_render_head& hd, const float* o, const float* __restrict__ rays,
+                                               const ff_samples& s, int64_t i, float* __restrict__ rgb) {
+  if constexpr (HEAD == kHeadFeat) {
+    float* f = hd.feat + i * hd.feat_stride;
+#pragma unroll
+    for (int c = 0; c < 31; ++c) f[c] = o[1 + c];
+  } else {
+    const int64_t rb = s.ray[i];
+    const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = sigmoidf(sh2_eval(o + 1 + 9 * c, dx, dy, dz));
+  }
+}
+
+template <int CK, bool BF, int HEAD>
 __global__ __launch_bounds__(256) void nerf_field_fwd_kernel(ff_field_desc fd, ff_mlp_desc m,
                                                              ff_render_head hd,
                                                              const float* __restrict__ rays,
@@ -117,24 +151,22 @@ __global__ __launch_bounds__(256) void nerf_field_fwd_kernel(ff_field_desc fd, f
                                                              float* __restrict__ sigma,
                                                              float* __restrict__ rgb) {
   constexpr int C = Spec18::C;
+  constexpr int O = head_out<HEAD>();
   const int64_t n = s.ray_off[n_rays];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     float xv[3] = {s.xyz[i * 3 + 0], s.xyz[i * 3 + 1], s.xyz[i * 3 + 2]};
-    const int64_t rb = s.ray[i];
-    const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
-    float bas[C], cof[C], f[C], h[64], o[28];
+    float bas[C], cof[C], f[C], h[64], o[O];
     coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
     combine<C, CK>(fd, bas, cof, f);
-    mlp_fwd<C, 64, 28, 2>(m, f, h, o);
+    mlp_fwd<C, 64, O, 2>(m, f, h, o);
     const float x0 = o[0] + hd.density_shift;
     sigma[i] = hd.density_act == 0 ? softplusf(x0) : fmaxf(x0, 0.f);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = sigmoidf(sh2_eval(o + 1 + 9 * c, dx, dy, dz));
+    head_out_store<HEAD>(hd, o, rays, s, i, rgb);
   }
 }
 
-template <int CK, bool BF>
+template <int CK, bool BF, int HEAD>
 __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff_mlp_desc m,
                                                             ff_render_head hd, ff_grads g,
                                                             const float* __restrict__ rays,
@@ -142,7 +174,8 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff
                                                             const float* __restrict__ d_sample,
                                                             float* __restrict__ ws) {
   constexpr int C = Spec18::C;
-  using Acc = MlpAcc<C, 64, 28, 2>;
+  constexpr int O = head_out<HEAD>();
+  using Acc = MlpAcc<C, 64, O, 2>;
   static_assert(kWave * 65 >= kStageWords, "LDS tile must also hold the scatter staging area");
   __shared__ float lds[kWave * 65];
   Acc acc;
@@ -162,22 +195,22 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff
     float bas[C], cof[C], f[C], df[C];
     coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
     combine<C, CK>(fd, bas, cof, f);
-    mlp_train_tile_fn<C, 64, 28, 2>(
+    mlp_train_tile_fn<C, 64, O, 2>(
         m, acc, lds, f, active,
-        [&](const float* o, float* dout) { nerf_dout(hd, o, gs, dx, dy, dz, dout); },
+        [&](const float* o, float* dout) { nerf_dout<HEAD>(hd, o, gs, dx, dy, dz, ic, dout); },
         df);
     float db[C], dc[C];
     combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
     coding_bwd_wave<3, CK, BF, Spec18>(fd, gl, xv, db, dc, active, lds);
   }
-  mlp_acc_store<C, 64, 28, 2>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
+  mlp_acc_store<C, 64, O, 2>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
 }
 
 // BF16-table forward with the matrix-core decoder: one wave per 64-sample tile.
 // Bounded to three waves per SIMD (168 VGPRs, a few spills): the gathers are
 // latency-bound, and the third wave hides more of them than the spills cost
 // (measured 70 -> 54 us per 196 K samples; four waves spill too much).
-template <int CK>
+template <int CK, int HEAD>
 __global__ __launch_bounds__(256, 3) void nerf_field_fwd_mfma_kernel(ff_field_desc fd, ff_mlp_desc m,
                                                                   ff_render_head hd,
                                                                   const float* __restrict__ rays,
@@ -185,13 +218,14 @@ __global__ __launch_bounds__(256, 3) void nerf_field_fwd_mfma_kernel(ff_field_de
                                                                   float* __restrict__ sigma,
                                                                   float* __restrict__ rgb) {
   constexpr int C = Spec18::C;
+  constexpr int O = head_out<HEAD>();
   const int lane = threadIdx.x & 63;
   const int64_t n = s.ray_off[n_rays];
   const int64_t ntiles = (n + kWave - 1) / kWave;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   if (wave >= ntiles) return;  // the grid is sized for the capacity, not the live count
-  MfmaDecoder W;
+  MfmaDecoder<O> W;
   W.load(m);
   for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
     const int64_t i = tile * kWave + lane;
@@ -204,12 +238,9 @@ __global__ __launch_bounds__(256, 3) void nerf_field_fwd_mfma_kernel(ff_field_de
     combine<C, CK>(fd, bas, cof, f);
     mfma_decoder_fwd(W, f, o);
     if (active) {
-      const int64_t rb = s.ray[i];
-      const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
       const float x0 = o[0] + hd.density_shift;
       sigma[i] = hd.density_act == 0 ? softplusf(x0) : fmaxf(x0, 0.f);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = sigmoidf(sh2_eval(o + 1 + 9 * c, dx, dy, dz));
+      head_out_store<HEAD>(hd, o, rays, s, i, rgb);
     }
   }
 }
@@ -217,7 +248,7 @@ __global__ __launch_bounds__(256, 3) void nerf_field_fwd_mfma_kernel(ff_field_de
 // BF16-table backward: gathers recomputed, matrix-core decoder forward and
 // backward, run-merging table scatter; decoder weight gradients accumulate in
 // the matrix-core accumulators across the wave's tiles.
-template <int CK>
+template <int CK, int HEAD>
 __global__ __launch_bounds__(64) void nerf_field_bwd_mfma_kernel(ff_field_desc fd, ff_mlp_desc m,
                                                                  ff_render_head hd, ff_grads g,
                                                                  const float* __restrict__ rays,
@@ -230,11 +261,12 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_mfma_kernel(ff_field_desc f
   constexpr int kValRow = 2 * C + 1;
   __shared__ __attribute__((aligned(16))) uint32_t lds[kDec + kWave * kValRow];
   float* vals = reinterpret_cast<float*>(lds + kDec) + threadIdx.x * kValRow;
-  MfmaDecoder W;
+  constexpr int O = head_out<HEAD>();
+  MfmaDecoder<O> W;
   W.load(m);
-  MfmaDecoderBwd WB;
+  MfmaDecoderBwd<O> WB;
   WB.load(m);
-  MfmaDecoderAcc acc;
+  MfmaDecoderAcc<O> acc;
   acc.zero();
   const int lane = threadIdx.x;
   const int64_t n = s.ray_off[n_rays];
@@ -260,9 +292,9 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_mfma_kernel(ff_field_desc f
       const int64_t rb = s.ray[ic];
       const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
       const float4 gs = reinterpret_cast<const float4*>(d_sample)[ic];
-      nerf_dout(hd, o, gs, dx, dy, dz, dout);
+      nerf_dout<HEAD>(hd, o, gs, dx, dy, dz, ic, dout);
 #pragma unroll
-      for (int k = 0; k < 32; ++k) dout[k] = (active && k < kMfO) ? dout[k] : 0.f;
+      for (int k = 0; k < 32; ++k) dout[k] = (active && k < O) ? dout[k] : 0.f;
     }
     mfma_decoder_bwd(WB, msk, dout, df, acc, lds);
     float bas[C], cof[C], db[C], dc[C];
@@ -274,11 +306,11 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_mfma_kernel(ff_field_desc f
     combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
     coding_bwd_wave<3, CK, true, Spec18, true>(fd, gl, xv, db, dc, active, reinterpret_cast<float*>(lds));
   }
-  acc.store(ws + (int64_t)blockIdx.x * (kMfH * kMfIn + kMfH + kMfO * kMfH));
+  acc.store(ws + (int64_t)blockIdx.x * (kMfH * kMfIn + kMfH + O * kMfH));
 }
 
 // compute_alpha (models/FactorFields.py:710-727) accumulated into alpha.
-template <int CK, bool BF>
+template <int CK, bool BF, int HEAD>
 __global__ __launch_bounds__(256) void nerf_alpha_kernel(ff_field_desc fd, ff_mlp_desc m, ff_render_head hd,
                                                          ff_ray_sampler occ, const float* __restrict__ xyz,
                                                          int64_t n, float length,
@@ -299,10 +331,11 @@ __global__ __launch_bounds__(256) void nerf_alpha_kernel(ff_field_desc fd, ff_ml
     for (int k = 0; k < 8; ++k) acc = acc + (occ.occ[cr.tex[k]] ? cr.w[k] : 0.f);
     if (!(acc > 0.f)) return;   // sigma = 0 -> alpha += 0
   }
-  float bas[C], cof[C], f[C], h[64], o[28];
+  constexpr int O = head_out<HEAD>();
+  float bas[C], cof[C], f[C], h[64], o[O];
   coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
   combine<C, CK>(fd, bas, cof, f);
-  mlp_fwd<C, 64, 28, 2>(m, f, h, o);
+  mlp_fwd<C, 64, O, 2>(m, f, h, o);
   const float x0 = o[0] + hd.density_shift;
   const float sg = hd.density_act == 0 ? softplusf(x0) : fmaxf(x0, 0.f);
   alpha[i] += 1.f - expf(-sg * length);
@@ -328,7 +361,7 @@ bool nerf_supported(const ff_field_desc& d, const ff_mlp_desc& m) {
     if (d.basis_dims[l] != dims[l]) return false;
   if (!(d.coeff_kind == FF_COEFF_GRID && d.coeff_ndim == 3) && d.coeff_kind != FF_COEFF_VM) return false;
   if (d.coeff_kind == FF_COEFF_VM && d.table_dtype != FF_F32) return false;
-  if (m.n_layers != 2 || m.in_dim != 18 || m.hidden != 64 || m.out_dim != 28) return false;
+  if (m.n_layers != 2 || m.in_dim != 18 || m.hidden != 64 || (m.out_dim != 28 && m.out_dim != 32)) return false;
   if (!m.w0 || !m.b0 || !m.w1) return false;
   for (int l = 0; l < 6; ++l)
     if (!d.basis[l] || d.basis_reso[l] < 2) return false;
@@ -338,11 +371,29 @@ bool nerf_supported(const ff_field_desc& d, const ff_mlp_desc& m) {
   return true;
 }
 
+// The head matches the decoder (SH: 28 outputs, features: 32) and carries
+// the buffers its pass needs.
+bool head_ok(const ff_render_head& h, const ff_mlp_desc& m, bool bwd) {
+  if (h.color == FF_COLOR_SH) return m.out_dim == 28;
+  if (h.color != FF_COLOR_FEATURES || m.out_dim != 32 || h.feat_stride < 31) return false;
+  return bwd ? (h.d_feat && h.app_pos) : (h.feat != nullptr);
+}
+
+template <int CK, bool BF, class F>
+int with_head(int head, F&& f) {
+  if (head == kHeadFeat)
+    return f(std::integral_constant<int, CK>{}, std::integral_constant<bool, BF>{},
+             std::integral_constant<int, kHeadFeat>{});
+  return f(std::integral_constant<int, CK>{}, std::integral_constant<bool, BF>{},
+           std::integral_constant<int, kHeadSH>{});
+}
+
+// f(CK, BF, HEAD) for the field's coefficient kind, table type and head.
 template <class F>
-int with_nerf(const ff_field_desc& d, F&& f) {
-  if (d.coeff_kind == FF_COEFF_VM) return f(std::integral_constant<int, CK_VM>{}, std::false_type{});
-  if (d.table_dtype == FF_BF16) return f(std::integral_constant<int, CK_GRID3>{}, std::true_type{});
-  return f(std::integral_constant<int, CK_GRID3>{}, std::false_type{});
+int with_nerf(const ff_field_desc& d, int head, F&& f) {
+  if (d.coeff_kind == FF_COEFF_VM) return with_head<CK_VM, false>(head, f);
+  if (d.table_dtype == FF_BF16) return with_head<CK_GRID3, true>(head, f);
+  return with_head<CK_GRID3, false>(head, f);
 }
 
 unsigned rf_blocks(int64_t cap, int per, int64_t per_cu) {
@@ -351,7 +402,7 @@ unsigned rf_blocks(int64_t cap, int per, int64_t per_cu) {
   return (unsigned)(need < lim ? (need > 0 ? need : 1) : lim);
 }
 
-constexpr int kNerfPartial = 64 * 18 + 64 + 28 * 64;
+constexpr int nerf_partial(int O) { return 64 * 18 + 64 + O * 64; }
 // Backward kernels: one wave per SIMD is resident (registers, LDS), so four
 // 64-thread blocks per CU; each block writes one decoder-gradient partial row.
 constexpr int kBwdBlocksPerCu = 4;
@@ -366,10 +417,10 @@ extern "C" {
 int ff_render_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,
                         const float* rays, int64_t n_rays, const ff_samples* s, float* sigma,
                         float* rgb, void* stream) {
-  if (!desc || !mlp || !head || !s || !s->ray_off || !s->xyz || !s->ray || !rays || !sigma || !rgb)
-    return FF_ERR_ARG;
-  if (n_rays < 0) return FF_ERR_ARG;
+  if (!desc || !mlp || !head || !s || !s->ray_off || !s->xyz || !s->ray || !rays || !sigma) return FF_ERR_ARG;
+  if (n_rays < 0 || (head->color == FF_COLOR_SH && !rgb)) return FF_ERR_ARG;
   if (!nerf_supported(*desc, *mlp)) return FF_ERR_UNSUPPORTED;
+  if (!head_ok(*head, *mlp, false)) return FF_ERR_ARG;
   if (s->cap <= 0 || n_rays == 0) return FF_OK;
   hipStream_t st = (hipStream_t)stream;
   const ff_field_desc d = *desc;
@@ -377,14 +428,15 @@ int ff_render_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
   const ff_render_head h = *head;
   const ff_samples sv = *s;
   const unsigned blocks = rf_blocks(s->cap, 256, 8);
-  return with_nerf(d, [&](auto CKi, auto BFi) {
+  return with_nerf(d, h.color == FF_COLOR_FEATURES ? kHeadFeat : kHeadSH, [&](auto CKi, auto BFi, auto Hi) {
     constexpr int CK = decltype(CKi)::value;
     constexpr bool BF = decltype(BFi)::value;
+    constexpr int HEAD = decltype(Hi)::value;
     if constexpr (BF)
-      hipLaunchKernelGGL((nerf_field_fwd_mfma_kernel<CK>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
+      hipLaunchKernelGGL((nerf_field_fwd_mfma_kernel<CK, HEAD>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
                          n_rays, sigma, rgb);
     else
-      hipLaunchKernelGGL((nerf_field_fwd_kernel<CK, BF>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
+      hipLaunchKernelGGL((nerf_field_fwd_kernel<CK, BF, HEAD>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
                          n_rays, sigma, rgb);
     return rf_status();
   });
@@ -406,11 +458,13 @@ int ff_render_alpha(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_
   const ff_field_desc d = *desc;
   const ff_mlp_desc m = *mlp;
   const ff_render_head h = *head;
-  return with_nerf(d, [&](auto CKi, auto BFi) {
+  // only the density is evaluated: the head selects the decoder width
+  return with_nerf(d, m.out_dim == 32 ? kHeadFeat : kHeadSH, [&](auto CKi, auto BFi, auto Hi) {
     constexpr int CK = decltype(CKi)::value;
     constexpr bool BF = decltype(BFi)::value;
-    hipLaunchKernelGGL((nerf_alpha_kernel<CK, BF>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, m,
-                       h, occ, xyz, n, length, alpha_accum);
+    constexpr int HEAD = decltype(Hi)::value;
+    hipLaunchKernelGGL((nerf_alpha_kernel<CK, BF, HEAD>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d,
+                       m, h, occ, xyz, n, length, alpha_accum);
     return rf_status();
   });
 }
@@ -418,7 +472,7 @@ int ff_render_alpha(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_
 size_t ff_render_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp,
                                            int64_t cap) {
   if (!desc || !mlp || cap <= 0 || !nerf_supported(*desc, *mlp)) return 0;
-  return ((size_t)rf_blocks(cap, 64, kBwdBlocksPerCu) + kReduceSplit) * kNerfPartial * 4;
+  return ((size_t)rf_blocks(cap, 64, kBwdBlocksPerCu) + kReduceSplit) * nerf_partial(mlp->out_dim) * 4;
 }
 
 }  // extern "C"
@@ -433,6 +487,7 @@ int render_field_bwd_impl(const ff_field_desc* desc, const ff_mlp_desc* mlp, con
     return FF_ERR_ARG;
   if (n_rays < 0) return FF_ERR_ARG;
   if (!nerf_supported(*desc, *mlp)) return FF_ERR_UNSUPPORTED;
+  if (!head_ok(*head, *mlp, true)) return FF_ERR_ARG;
   for (int l = 0; l < 6; ++l)
     if (!grads->basis[l]) return FF_ERR_ARG;
   const int nc = desc->coeff_kind == FF_COEFF_VM ? 6 : 1;
@@ -441,26 +496,27 @@ int render_field_bwd_impl(const ff_field_desc* desc, const ff_mlp_desc* mlp, con
   if (!grads->w0 || !grads->b0 || !grads->w1) return FF_ERR_ARG;
   if (s->cap <= 0 || n_rays == 0) return FF_OK;
   const unsigned blocks = rf_blocks(s->cap, 64, kBwdBlocksPerCu);
-  if (!ws || ws_bytes < ((size_t)blocks + kReduceSplit) * kNerfPartial * 4) return FF_ERR_WORKSPACE;
+  if (!ws || ws_bytes < ((size_t)blocks + kReduceSplit) * nerf_partial(mlp->out_dim) * 4) return FF_ERR_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   const ff_field_desc d = *desc;
   const ff_mlp_desc m = *mlp;
   const ff_render_head h = *head;
   const ff_samples sv = *s;
   const ff_grads g = *grads;
-  int r = with_nerf(d, [&](auto CKi, auto BFi) {
+  int r = with_nerf(d, h.color == FF_COLOR_FEATURES ? kHeadFeat : kHeadSH, [&](auto CKi, auto BFi, auto Hi) {
     constexpr int CK = decltype(CKi)::value;
     constexpr bool BF = decltype(BFi)::value;
+    constexpr int HEAD = decltype(Hi)::value;
     if constexpr (BF)
-      hipLaunchKernelGGL((nerf_field_bwd_mfma_kernel<CK>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays, sv,
-                         n_rays, d_sample, (float*)ws);
+      hipLaunchKernelGGL((nerf_field_bwd_mfma_kernel<CK, HEAD>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays,
+                         sv, n_rays, d_sample, (float*)ws);
     else
-      hipLaunchKernelGGL((nerf_field_bwd_kernel<CK, BF>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays, sv,
+      hipLaunchKernelGGL((nerf_field_bwd_kernel<CK, BF, HEAD>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays, sv,
                          n_rays, d_sample, (float*)ws);
     return rf_status();
   });
   if (r) return r;
-  return launch_reduce_partials((const float*)ws, (int)blocks, 2, 18, 64, 28, g, st);
+  return launch_reduce_partials((const float*)ws, (int)blocks, 2, 18, 64, m.out_dim, g, st);
 }
 
 // Scratch of ff_render_field_bwd_noalloc: the decoder-gradient partial rows

@@ -70,8 +70,26 @@ class Samples(ctypes.Structure):
                 ("ray_off", c_vp), ("cap", ctypes.c_int64), ("coding", c_vp)]
 
 
+COLOR_SH, COLOR_FEATURES = 0, 1
+
+
 class RenderHead(ctypes.Structure):
-    _fields_ = [("density_act", c_i32), ("density_shift", c_f32)]
+    _fields_ = [("density_act", c_i32), ("density_shift", c_f32), ("color", c_i32), ("feat_stride", c_i32),
+                ("feat", c_vp), ("d_feat", c_vp), ("app_pos", c_vp)]
+
+
+class ColorMlpDesc(ctypes.Structure):
+    _fields_ = [("in_feat", c_i32), ("view_pe", c_i32), ("fea_pe", c_i32), ("hidden", c_i32), ("n_layers", c_i32),
+                ("w0", c_vp), ("b0", c_vp), ("w1", c_vp), ("b1", c_vp), ("w2", c_vp)]
+
+
+class ColorRows(ctypes.Structure):
+    _fields_ = [("feat", c_vp), ("feat_stride", ctypes.c_int64), ("dirs", c_vp), ("dir_stride", ctypes.c_int64),
+                ("dir_index", c_vp), ("rows", c_vp), ("count_dev", c_vp), ("n_max", ctypes.c_int64)]
+
+
+class ColorMlpGrads(ctypes.Structure):
+    _fields_ = [("w0", c_vp), ("b0", c_vp), ("w1", c_vp), ("b1", c_vp), ("w2", c_vp)]
 
 
 class ProposalDesc(ctypes.Structure):
@@ -125,6 +143,12 @@ def _declare(lib):
                                                        P(Samples), c_vp, P(Grads), c_vp]),
         "ff_render_alpha": (ctypes.c_int, [P(FieldDesc), P(MlpDesc), P(RenderHead), P(RaySampler), c_vp,
                                            I64, c_f32, c_vp, c_vp]),
+        "ff_select_app": (ctypes.c_int, [P(Samples), I64, c_vp, c_f32, c_vp, c_vp, c_vp, c_vp]),
+        "ff_color_mlp_workspace_bytes": (sz, [P(ColorMlpDesc), I64, ctypes.c_int]),
+        "ff_color_mlp_fwd": (ctypes.c_int, [P(ColorMlpDesc), P(ColorRows), c_vp, I64, c_vp, sz, c_vp]),
+        "ff_color_mlp_bwd": (ctypes.c_int, [P(ColorMlpDesc), P(ColorRows), c_vp, I64, c_vp, I64, P(ColorMlpGrads),
+                                            c_vp, sz, c_vp]),
+        "ff_dropout": (ctypes.c_int, [c_vp, I64, c_f32, c_vp, ctypes.c_uint64, ctypes.c_uint64, c_vp, c_vp, c_vp]),
         "ff_occupancy_pool": (ctypes.c_int, [c_vp, c_i32, c_i32, c_i32, c_f32, c_f32, c_vp, c_vp]),
         "ff_proposal_sample": (ctypes.c_int, [P(ProposalDesc), c_vp, c_vp, c_vp, I64, P(Samples), c_vp]),
         "ff_proposal_bwd_workspace_bytes": (ctypes.c_size_t, [I64]),

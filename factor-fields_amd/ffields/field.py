@@ -22,7 +22,7 @@ import torch.nn as nn
 
 from . import _lib as L
 from .config import AttrDict
-from .ops import Field, FieldSpec, GetCoding, LinearMat, is_texel_major, mlp_desc, texel_major
+from .ops import Dropout, Field, FieldSpec, GetCoding, LinearMat, is_texel_major, mlp_desc, texel_major
 from .params import N_to_reso, dct_basis, setup_params, vm_reso
 from .render import AlphaGridMask, RenderMixin
 
@@ -32,7 +32,10 @@ VEC_MODE = [2, 1, 0]                  # models/FactorFields.py:214
 
 class MLPMixer(nn.Module):
     """linear_mat (models/FactorFields.py:113-159) with pe = 0; same state_dict
-    keys (backbone.{l}.weight / .bias)."""
+    keys (backbone.{l}.weight / .bias).  With with_dropout, training calls
+    (is_train=True) drop input entries with p = 0.1 (:150-151) on the device;
+    ``keep`` (bool / 0-1, the shape of x) injects the mask instead of drawing
+    it."""
 
     def __init__(self, in_dim, out_dim=16, num_layers=2, hidden_dim=64, pe=0, with_dropout=False):
         super().__init__()
@@ -57,9 +60,13 @@ class MLPMixer(nn.Module):
             return [self.backbone[0].weight]
         return [self.backbone[0].weight, self.backbone[0].bias, self.backbone[1].weight]
 
-    def forward(self, x, is_train=False):
+    dropout_p = 0.1
+
+    def forward(self, x, is_train=False, keep=None):
         if self.with_dropout and is_train:
-            raise NotImplementedError("linear_mat input dropout has no HIP kernel")
+            # a fresh draw per call, seeded from torch's host generator (no device sync)
+            seed = int(torch.randint(0, 2 ** 62, (1,)).item()) if keep is None else 0
+            x = Dropout.apply(x, self.dropout_p, seed, keep)
         return LinearMat.apply(x, *self.params_list())
 
 

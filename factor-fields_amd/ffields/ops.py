@@ -255,3 +255,30 @@ class Field(torch.autograd.Function):
         for k, gr in enumerate(gp + gt):
             out.append(gr if needs[4 + k] else None)
         return tuple(out)
+
+
+class Dropout(torch.autograd.Function):
+    """torch.nn.functional.dropout(x, p) as linear_mat applies it to its input
+    (models/FactorFields.py:150-151): out = x * keep / (1 - p).  The mask is
+    `keep` (0/1 uint8, the shape of x) when given, else drawn on the device by
+    ff_dropout's counter-based generator from `seed`; the backward applies
+    the same mask to the output gradient (regenerated from the seed, not
+    stored)."""
+
+    @staticmethod
+    def forward(ctx, x, p, seed, keep=None):
+        x = _f32c(x)
+        out = torch.empty_like(x)
+        k = None if keep is None else keep.to(torch.uint8).contiguous()
+        L.check(L.lib().ff_dropout(L.ptr(x), x.numel(), float(p), L.ptr(k), int(seed), 0, L.ptr(out), None,
+                                   L.stream_of(x)), "ff_dropout")
+        ctx.p, ctx.seed, ctx.keep = p, seed, k
+        return out
+
+    @staticmethod
+    def backward(ctx, d_out):
+        d_out = _f32c(d_out)
+        dx = torch.empty_like(d_out)
+        L.check(L.lib().ff_dropout(L.ptr(d_out), d_out.numel(), float(ctx.p), L.ptr(ctx.keep), int(ctx.seed), 0,
+                                   L.ptr(dx), None, L.stream_of(d_out)), "ff_dropout")
+        return dx, None, None, None

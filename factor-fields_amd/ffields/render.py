@@ -1,11 +1,15 @@
 """Volume rendering on the HIP kernels: FactorFields.forward
 (models/FactorFields.py:843-898) for bounded scenes, with the sampler
-(:586-602), AlphaGridMask occupancy (:91-110), field + density/SH head,
-compositing (:82-88, :878-896) and their backward passes.
+(:586-602), AlphaGridMask occupancy (:91-110), field + density head,
+colour module, compositing (:82-88, :878-896) and their backward passes.
 
-The colour head is degree-2 spherical harmonics (models/sh.py:34-64) on
-decoder outputs 1..27 followed by a sigmoid; it is evaluated inside the field
-kernel, so ``renderModule`` holds no parameters.
+Two colour modules (cfg.renderer.shadingMode):
+- ``MLP_Fea``: the reference's own MLPRender_Fea (:162-203) on decoder
+  features 1..31, evaluated only on the app samples (weight >
+  rayMarch_weight_thres, :878-885) by ff_color_mlp_fwd / ff_color_mlp_bwd;
+- ``SH``: degree-2 spherical harmonics (models/sh.py:34-64) on decoder
+  outputs 1..27 followed by a sigmoid, evaluated inside the field kernel
+  (``renderModule`` then holds no parameters).
 """
 
 import ctypes
@@ -47,6 +51,90 @@ class SHRender(nn.Module):
         raise NotImplementedError("the SH colour head is evaluated inside ff_render_field_fwd")
 
 
+class MLPRender_Fea(nn.Module):
+    """models/FactorFields.py:162-203 on the HIP colour kernels (FP32 matrix
+    cores), with the reference's state_dict keys (mlp.{l}.weight / .bias).
+    Built for inChanel 31, fea_pe 2, view_pe 6 or 2, hidden 128, 3 layers (the
+    reference's configs); other shapes raise NotImplementedError."""
+
+    def __init__(self, inChanel, num_layers=3, hidden_dim=64, viewpe=6, feape=2):
+        super().__init__()
+        if (inChanel, num_layers, hidden_dim, feape) != (31, 3, 128, 2) or viewpe not in (6, 2):
+            raise NotImplementedError(
+                f"MLPRender_Fea(inChanel={inChanel}, num_layers={num_layers}, hidden_dim={hidden_dim}, "
+                f"viewpe={viewpe}, feape={feape}) has no HIP kernel (built: 31, 3, 128, 6|2, 2)")
+        self.in_mlpC = 3 + inChanel + 2 * viewpe * 3 + 2 * feape * inChanel
+        self.num_layers = num_layers
+        self.viewpe = viewpe
+        self.feape = feape
+        self.inChanel = inChanel
+        mlp = []
+        for l in range(num_layers):
+            in_dim = self.in_mlpC if l == 0 else hidden_dim
+            out_dim, bias = (3, False) if l == num_layers - 1 else (hidden_dim, True)
+            mlp.append(nn.Linear(in_dim, out_dim, bias=bias))
+        self.mlp = nn.ModuleList(mlp)
+
+    def params_list(self):
+        return [self.mlp[0].weight, self.mlp[0].bias, self.mlp[1].weight, self.mlp[1].bias, self.mlp[2].weight]
+
+    def desc(self):
+        d = L.ColorMlpDesc()
+        d.in_feat, d.view_pe, d.fea_pe, d.hidden, d.n_layers = self.inChanel, self.viewpe, self.feape, 128, 3
+        p = self.params_list()
+        d.w0, d.b0, d.w1, d.b1, d.w2 = (t.data_ptr() for t in p)
+        return d
+
+    def grads_struct(self, grads):
+        g = L.ColorMlpGrads()
+        g.w0, g.b0, g.w1, g.b1, g.w2 = (t.data_ptr() for t in grads)
+        return g
+
+    def forward(self, viewdirs, features):
+        return ColorMLP.apply(self, viewdirs, features, *self.params_list())
+
+
+def _color_ws(module, n_max, backward, device, cache=None):
+    nb = L.lib().ff_color_mlp_workspace_bytes(ctypes.byref(module.desc()), n_max, int(backward))
+    if cache is not None:
+        return cache(nb), nb
+    return torch.empty((max(nb, 4) + 3) // 4, device=device), nb
+
+
+class ColorMLP(torch.autograd.Function):
+    """renderModule(viewdirs, features) -> rgb [N, 3] with the hand-written
+    backward (feature and weight gradients; view directions get none)."""
+
+    @staticmethod
+    def forward(ctx, module, viewdirs, features, *params):
+        v = viewdirs.contiguous().float()
+        f = features.contiguous().float()
+        n = f.shape[0]
+        rgb = torch.empty((n, 3), device=f.device)
+        r = L.ColorRows(f.data_ptr(), f.shape[1], v.data_ptr(), 3, None, None, None, n)
+        ws, nb = _color_ws(module, n, False, f.device)
+        L.check(L.lib().ff_color_mlp_fwd(ctypes.byref(module.desc()), ctypes.byref(r), L.ptr(rgb), 3, L.ptr(ws), nb,
+                                         L.stream_of(f)), "ff_color_mlp_fwd")
+        ctx.module = module
+        ctx.save_for_backward(v, f, *params)
+        return rgb
+
+    @staticmethod
+    def backward(ctx, d_rgb):
+        v, f, *params = ctx.saved_tensors
+        module = ctx.module
+        n = f.shape[0]
+        d_feat = torch.empty_like(f)
+        gp = [torch.zeros_like(p) for p in params]
+        r = L.ColorRows(f.data_ptr(), f.shape[1], v.data_ptr(), 3, None, None, None, n)
+        ws, nb = _color_ws(module, n, True, f.device)
+        L.check(L.lib().ff_color_mlp_bwd(ctypes.byref(module.desc()), ctypes.byref(r), L.ptr(d_rgb.contiguous()), 3,
+                                         L.ptr(d_feat), f.shape[1], ctypes.byref(module.grads_struct(gp)), L.ptr(ws),
+                                         nb, L.stream_of(f)), "ff_color_mlp_bwd")
+        needs = ctx.needs_input_grad
+        return (None, None, d_feat if needs[2] else None) + tuple(g if nd else None for g, nd in zip(gp, needs[3:]))
+
+
 def _samples_struct(buf):
     s = L.Samples()
     s.xyz = buf["xyz"].data_ptr()
@@ -77,11 +165,21 @@ class RenderMixin:
         cfg = self.cfg
         if cfg.dataset.get("is_unbound", False):
             raise NotImplementedError("unbounded scenes have no HIP sampler")
-        if cfg.renderer.get("shadingMode", "SH") not in ("SH",):
-            raise NotImplementedError("only the degree-2 SH colour head is built")
-        if cfg.model.out_dim != 28:
-            raise ValueError("the SH colour head needs out_dim = 1 + 27")
-        self.renderModule = SHRender()
+        mode = cfg.renderer.get("shadingMode", "MLP_Fea")   # the reference always builds MLPRender_Fea
+        if mode == "MLP_Fea":
+            # models/FactorFields.py:243-245
+            self.renderModule = MLPRender_Fea(inChanel=cfg.model.out_dim - 1, num_layers=cfg.renderer.num_layers,
+                                              hidden_dim=cfg.renderer.hidden_dim, viewpe=cfg.renderer.view_pe,
+                                              feape=cfg.renderer.fea_pe).to(self.device)
+            if cfg.model.out_dim != 32:
+                raise NotImplementedError("the feature decoder of the render kernels has out_dim 32")
+        elif mode == "SH":
+            if cfg.model.out_dim != 28:
+                raise ValueError("the SH colour head needs out_dim = 1 + 27")
+            self.renderModule = SHRender()
+        else:
+            raise NotImplementedError(f"shadingMode {mode!r} has no HIP kernel (built: MLP_Fea, SH)")
+        self._feat_head = mode == "MLP_Fea"
         self.is_unbound = False
         self.inward_aabb = self.aabb
         self.cur_volumeSize = N_to_reso(cfg.training.volume_resoInit ** self.in_dim, self.aabb.cpu())
@@ -96,6 +194,17 @@ class RenderMixin:
         aabbDiag = torch.sqrt(torch.sum(torch.square(self.aabbSize)))
         self.nSamples = int((aabbDiag / self.stepSize).item()) + 1
         self._step = float(self.stepSize.item())
+
+    @torch.no_grad()
+    def upsample_volume_grid(self, res_target):
+        """models/FactorFields.py:701-708: new render parameters for the grid
+        resolution res_target (step size and sample count).  The reference
+        also resizes the coefficient grid for its multi-scene google_objs
+        loader, which is not part of this library."""
+        self.update_renderParams(res_target)
+        if (self.cfg.dataset.get("dataset_name") == "google_objs" and self.n_scene == 1
+                and self.coeff_type == "grid"):
+            raise NotImplementedError("coefficient upsampling of the google_objs loader is not built")
 
     def _aabb_host(self):
         """Host copy of the box (the kernels take it by value); refreshed only
@@ -124,10 +233,20 @@ class RenderMixin:
                 s.occ_origin[k], s.occ_inv[k] = am._origin[k], am._inv[k]
         return s
 
-    def _head(self):
+    def _head(self, buf=None):
+        """Density head, and with the MLP_Fea colour module the feature rows of
+        `buf` (written by the field forward) and their gradients (read by the
+        field backward through app_pos)."""
         h = L.RenderHead()
         h.density_act = 0 if self.cfg.renderer.fea2denseAct == "softplus" else 1
         h.density_shift = float(self.cfg.renderer.density_shift)
+        if getattr(self, "_feat_head", False):
+            h.color = L.COLOR_FEATURES
+            h.feat_stride = 32
+            if buf is not None:
+                h.feat = buf["feat"].data_ptr()
+                h.d_feat = buf["d_feat"].data_ptr()
+                h.app_pos = buf["app_pos"].data_ptr()
         return h
 
     def _field_structs(self):
@@ -156,23 +275,47 @@ class RenderMixin:
                        d_sample=torch.empty((cap, 4), device=dev),
                        coding=(torch.empty((2 * self._spec.C, cap), device=dev)
                                if self.table_dtype == "bf16" else None))
+            self._feature_buffers(buf, B)
             cache[key] = buf
         self._rbuf = (key, cache[key])
         return cache[key]
 
+    def _feature_buffers(self, buf, B):
+        """Per-sample buffers of the MLP_Fea colour module: decoder features
+        [cap, 32], their gradients (compact, one row per app sample), the app
+        list, each sample's position in it, and the per-ray app offsets."""
+        if not getattr(self, "_feat_head", False):
+            return buf
+        cap = buf["xyz"].shape[0]
+        dev = buf["xyz"].device
+        buf.update(feat=torch.empty((cap, 32), device=dev), d_feat=torch.empty((cap, 32), device=dev),
+                   app_rows=torch.empty(cap, device=dev, dtype=torch.int32),
+                   app_pos=torch.empty(cap, device=dev, dtype=torch.int32),
+                   app_off=torch.empty(B + 1, device=dev, dtype=torch.int32))
+        return buf
+
+    def _color_rows(self, rays, buf):
+        cap = buf["xyz"].shape[0]
+        B = rays.shape[0]
+        return L.ColorRows(buf["feat"].data_ptr(), 32, rays.data_ptr() + 12, 6, buf["ray"].data_ptr(),
+                           buf["app_rows"].data_ptr(), buf["app_off"].data_ptr() + 4 * B, cap)
+
+    def _color_workspace(self, nbytes):
+        ws = getattr(self, "_cws", None)
+        if ws is None or ws.numel() * 4 < nbytes:
+            ws = torch.empty((max(nbytes, 4) + 3) // 4, device=self.device, dtype=torch.float32)
+            self._cws = ws
+        return ws
+
     def render_samples(self, rays, n_samples, jitter, buf):
-        """Sampler + field + head over `rays`; fills buf (packed samples,
+        """Sampler + field + colour over `rays`; fills buf (packed samples,
         sigma, rgb).  Returns the ff_samples struct."""
         s = _samples_struct(buf)
         st = L.stream_of(rays)
         smp = self._sampler(n_samples)
         L.check(L.lib().ff_sample_rays(ctypes.byref(smp), L.ptr(rays), L.ptr(jitter), rays.shape[0],
                                        ctypes.byref(s), st), "ff_sample_rays")
-        d, m = self._field_structs()
-        h = self._head()
-        L.check(L.lib().ff_render_field_fwd(ctypes.byref(d), ctypes.byref(m), ctypes.byref(h), L.ptr(rays),
-                                            rays.shape[0], ctypes.byref(s), L.ptr(buf["sigma"]),
-                                            L.ptr(buf["rgb"]), st), "ff_render_field_fwd")
+        self._field_fwd(rays, s, buf, st)
         return s
 
 
@@ -192,10 +335,28 @@ class RenderMixin:
         rays = rays_chunk.contiguous().float()
         jit = self._jitter(rays, is_train, jitter)
         bg = bool(white_bg or (is_train and torch.rand((1,)) < 0.5))
+        if is_train and self.cfg.model.get("with_dropout", False):
+            raise NotImplementedError("linear_mat input dropout inside the render kernels is not built")
         params = self.linear_mat.params_list() + list(self._tables()[0]) + list(self._tables()[1])
+        if getattr(self, "_feat_head", False):
+            params = params + self.renderModule.params_list()
         rgb_map, depth = RenderRays.apply(self, rays, jit, n_samples, bg, *params)
-        coeffs = torch.zeros((1, sum(self.basis_dims)), device=rays.device)
-        return rgb_map, depth, coeffs
+        return rgb_map, depth, self._render_coeffs(is_train)
+
+    def _render_coeffs(self, is_train):
+        """The coeffs FactorFields.forward returns (models/FactorFields.py:
+        871-875): get_coding's coefficient values at the valid samples of the
+        last render, zeros (1, C) when there were none.  Only computed for
+        training renders, the only ones render_ray keeps (renderer.py:18-21);
+        it reads the sample count on the host."""
+        C = sum(self.basis_dims)
+        buf = getattr(self, "_last_render_buf", None)
+        if not is_train or buf is None:
+            return torch.zeros((1, C), device=self.device)
+        n = int(buf["ray_off"][-1])
+        if n == 0 or self.coeff_type == "none":
+            return torch.zeros((1, C), device=self.device)
+        return self.get_coeff(buf["xyz"][:n])
 
     # ------------------------------------------------ learned proposal sampler
     def enable_proposal(self, reso=64, n_fine=48, table_dtype="bf16", density_shift=-5.0, seed=0):
@@ -222,11 +383,37 @@ class RenderMixin:
                                          L.ptr(weights), stream), "ff_composite_fwd")
 
     def _field_fwd(self, rays, s, buf, stream):
+        """Field + density head over the packed samples, then the colour: the
+        SH head inside the field kernel, or (MLP_Fea) the app-sample selection
+        (models/FactorFields.py:878-882) and the colour module on those
+        samples (:883-885); rgb of the other samples is never read."""
         d, m = self._field_structs()
-        h = self._head()
+        h = self._head(buf)
         L.check(L.lib().ff_render_field_fwd(ctypes.byref(d), ctypes.byref(m), ctypes.byref(h), L.ptr(rays),
                                             rays.shape[0], ctypes.byref(s), L.ptr(buf["sigma"]),
                                             L.ptr(buf["rgb"]), stream), "ff_render_field_fwd")
+        if getattr(self, "_feat_head", False):
+            B = rays.shape[0]
+            thres = float(self.cfg.renderer.rayMarch_weight_thres)
+            L.check(L.lib().ff_select_app(ctypes.byref(s), B, L.ptr(buf["sigma"]), thres, L.ptr(buf["app_off"]),
+                                          L.ptr(buf["app_rows"]), L.ptr(buf["app_pos"]), stream), "ff_select_app")
+            rm = self.renderModule
+            r = self._color_rows(rays, buf)
+            nb = L.lib().ff_color_mlp_workspace_bytes(ctypes.byref(rm.desc()), r.n_max, 0)
+            ws = self._color_workspace(nb)
+            L.check(L.lib().ff_color_mlp_fwd(ctypes.byref(rm.desc()), ctypes.byref(r), L.ptr(buf["rgb"]), 3,
+                                             L.ptr(ws), nb, stream), "ff_color_mlp_fwd")
+
+    def _color_bwd(self, rays, buf, grads, stream):
+        """MLP_Fea backward on the app samples: dL/drgb from d_sample ->
+        compact feature gradients (d_feat) and the colour weights' grads."""
+        rm = self.renderModule
+        r = self._color_rows(rays, buf)
+        nb = L.lib().ff_color_mlp_workspace_bytes(ctypes.byref(rm.desc()), r.n_max, 1)
+        ws = self._color_workspace(nb)
+        L.check(L.lib().ff_color_mlp_bwd(ctypes.byref(rm.desc()), ctypes.byref(r), buf["d_sample"].data_ptr() + 4, 4,
+                                         L.ptr(buf["d_feat"]), 32, ctypes.byref(rm.grads_struct(grads)), L.ptr(ws), nb,
+                                         stream), "ff_color_mlp_bwd")
 
     def _train_from_samples(self, rays, target, s, buf, bg, loss_scale, loss_sum, weights=None):
         """composite forward (+weights) + MSE backward + field backward."""
@@ -240,9 +427,11 @@ class RenderMixin:
                                          int(bg), thres, L.ptr(rgb_raw), None, L.ptr(target),
                                          float(loss_scale), L.ptr(loss_sum), L.ptr(buf["d_sample"]), st),
                 "ff_composite_bwd")
-        d, m = self._field_structs()
-        h = self._head()
         g = self._train_grads()
+        if getattr(self, "_feat_head", False):
+            self._color_bwd(rays, buf, [p.grad for p in self.renderModule.params_list()], st)
+        d, m = self._field_structs()
+        h = self._head(buf)
         ws_bytes = L.lib().ff_render_field_bwd_workspace_bytes(ctypes.byref(d), ctypes.byref(m), s.cap)
         ws = self._workspace(ws_bytes)
         ev = getattr(self, "kernel_events", None)
@@ -314,6 +503,7 @@ class RenderMixin:
                     "ff_sample_rays")
         cap = buf["xyz"].shape[0]
         buf.update(sigma=torch.empty(cap, device=rays.device), rgb=torch.empty((cap, 3), device=rays.device))
+        self._feature_buffers(buf, B)
         self._field_fwd(rays, s, buf, st)
         rgb_map = torch.empty((B, 3), device=rays.device)
         rgb_raw = torch.empty((B, 3), device=rays.device)
@@ -482,6 +672,7 @@ class RenderRays(torch.autograd.Function):
         buf = alloc_samples(B, n_samples, dev)
         cap = buf["xyz"].shape[0]
         buf.update(sigma=torch.empty(cap, device=dev), rgb=torch.empty((cap, 3), device=dev))
+        model._feature_buffers(buf, B)
         s = model.render_samples(rays, n_samples, jitter, buf)
         rgb_map = torch.empty((B, 3), device=dev)
         rgb_raw = torch.empty((B, 3), device=dev)
@@ -491,6 +682,7 @@ class RenderRays(torch.autograd.Function):
                                          int(white_bg), thres, L.ptr(rgb_map), L.ptr(rgb_raw),
                                          L.ptr(depth), None, None, L.stream_of(rays)), "ff_composite_fwd")
         ctx.model, ctx.white_bg, ctx.thres, ctx.buf = model, white_bg, thres, buf
+        model._last_render_buf = buf
         ctx.save_for_backward(rays, rgb_raw)
         ctx.mark_non_differentiable(depth)
         return rgb_map, depth
@@ -503,13 +695,17 @@ class RenderRays(torch.autograd.Function):
         s = _samples_struct(buf)
         st = L.stream_of(rays)
         cap = buf["xyz"].shape[0]
-        d_sample = torch.empty((cap, 4), device=rays.device)
+        d_sample = buf["d_sample"] = torch.empty((cap, 4), device=rays.device)
         L.check(L.lib().ff_composite_bwd(ctypes.byref(s), B, L.ptr(buf["sigma"]), L.ptr(buf["rgb"]),
                                          int(ctx.white_bg), ctx.thres, L.ptr(rgb_raw),
                                          L.ptr(d_rgb.contiguous()), None, 1.0, None, L.ptr(d_sample), st),
                 "ff_composite_bwd")
+        gc = []
+        if getattr(model, "_feat_head", False):
+            gc = [torch.zeros_like(t) for t in model.renderModule.params_list()]
+            model._color_bwd(rays, buf, gc, st)
         d, m = model._field_structs()
-        h = model._head()
+        h = model._head(buf)
         p = model.linear_mat.params_list()
         b, c = model._tables()
         gp = [torch.zeros_like(t) for t in p]
@@ -526,7 +722,7 @@ class RenderRays(torch.autograd.Function):
                                             B, ctypes.byref(s), L.ptr(d_sample), ctypes.byref(g), L.ptr(ws),
                                             ws_bytes, st), "ff_render_field_bwd")
         needs = ctx.needs_input_grad[5:]
-        grads = [gr if nd else None for gr, nd in zip(gp + gt, needs)]
+        grads = [gr if nd else None for gr, nd in zip(gp + gt + gc, needs)]
         return (None, None, None, None, None) + tuple(grads)
 
 

@@ -253,21 +253,36 @@ int ff_sample_rays(const ff_ray_sampler* sampler, const float* rays, const float
                    int64_t n_rays, const ff_samples* out, void* stream);
 
 /* Density / colour head of the NeRF field (models/FactorFields.py:639-643,
- * basis2density; degree-2 spherical harmonics of models/sh.py:34-64 on
- * decoder outputs 1..27 followed by a sigmoid, as MLPRender_Fea's output). */
+ * basis2density).  color = FF_COLOR_SH: degree-2 spherical harmonics of
+ * models/sh.py:34-64 on decoder outputs 1..27 followed by a sigmoid (decoder
+ * out_dim 28).  color = FF_COLOR_FEATURES: the decoder outputs 1..out_dim-1
+ * are the features the reference hands to its colour module
+ * (feat[app_mask, 1:], models/FactorFields.py:883-885; MLPRender_Fea below),
+ * with out_dim 32. */
+enum ff_color_kind { FF_COLOR_SH = 0, FF_COLOR_FEATURES = 1 };
 typedef struct ff_render_head {
   int32_t density_act;   /* 0 softplus, 1 relu */
   float density_shift;   /* cfg.renderer.density_shift */
+  int32_t color;         /* enum ff_color_kind */
+  int32_t feat_stride;   /* FEATURES: floats between the feature rows below (>= out_dim - 1) */
+  float* feat;           /* FEATURES, forward: [cap, feat_stride] rows written per sample */
+  const float* d_feat;   /* FEATURES, backward: rows of dL/dfeatures, indexed by app_pos */
+  const int32_t* app_pos;/* FEATURES, backward: [cap] row of each sample in d_feat, -1 = no
+                          * colour gradient (not an app sample) */
 } ff_render_head;
 
 /* Field + head over the packed samples (count read on the device):
- * sigma [cap], rgb [cap, 3].  rays supplies each sample's view direction. */
+ * sigma [cap], and rgb [cap, 3] (FF_COLOR_SH; ignored with FF_COLOR_FEATURES,
+ * which writes head->feat instead).  rays supplies each sample's view
+ * direction. */
 int ff_render_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp,
                         const ff_render_head* head, const float* rays, int64_t n_rays,
                         const ff_samples* s, float* sigma, float* rgb, void* stream);
 size_t ff_render_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp,
                                            int64_t cap);
-/* d_sample [cap, 4] = (dL/dsigma, dL/drgb[3]) per sample -> table and decoder grads. */
+/* d_sample [cap, 4] = (dL/dsigma, dL/drgb[3]) per sample -> table and decoder
+ * grads.  With FF_COLOR_FEATURES only dL/dsigma is read from d_sample; the
+ * feature gradients come from head->d_feat / head->app_pos. */
 int ff_render_field_bwd(const ff_field_desc* desc, const ff_mlp_desc* mlp,
                         const ff_render_head* head, const float* rays, int64_t n_rays,
                         const ff_samples* s, const float* d_sample, const ff_grads* grads, void* ws,
@@ -308,6 +323,87 @@ int ff_composite_bwd(const ff_samples* s, int64_t n_rays, const float* sigma, co
                      int white_bg, float weight_thres, const float* rgb_raw,
                      const float* d_rgb_map, const float* target, float loss_scale,
                      float* loss_sum, float* d_sample, void* stream);
+
+/* App samples of FactorFields.forward (models/FactorFields.py:878-885): the
+ * samples whose compositing weight (raw2alpha :82-88 on sigma and s->dist)
+ * exceeds weight_thres, the only ones the colour module evaluates.  One
+ * wavefront per ray with the arithmetic of ff_composite_fwd, so both agree on
+ * every sample.  Outputs (device): app_off [n_rays + 1] exclusive offsets per
+ * ray (app_off[n_rays] = number of app samples), rows [cap] the app samples'
+ * indices in ray-major order, app_pos [cap] (may be NULL) the position of
+ * each live sample in rows or -1. */
+int ff_select_app(const ff_samples* s, int64_t n_rays, const float* sigma, float weight_thres,
+                  int32_t* app_off, int32_t* rows, int32_t* app_pos, void* stream);
+
+/* ====================================================== colour module ==
+ * MLPRender_Fea (models/FactorFields.py:162-203): the input is
+ * [features, viewdir, PE(features, fea_pe), PE(viewdir, view_pe)] with
+ * positional_encoding :74-79 (channel-major sin block, then the cos block),
+ * followed by Linear(in_mlpC, hidden) + ReLU, Linear(hidden, hidden) + ReLU,
+ * Linear(hidden, 3, no bias) and a sigmoid.  FP32 weights in torch layout.
+ * Built for in_feat 31, fea_pe 2, view_pe 6 or 2, hidden 128, 3 layers (the
+ * reference's configs); FF_ERR_UNSUPPORTED otherwise. */
+typedef struct ff_color_mlp_desc {
+  int32_t in_feat;       /* inChanel */
+  int32_t view_pe;
+  int32_t fea_pe;
+  int32_t hidden;
+  int32_t n_layers;
+  const float* w0;       /* [hidden, in_mlpC] */
+  const float* b0;       /* [hidden] */
+  const float* w1;       /* [hidden, hidden] */
+  const float* b1;       /* [hidden] */
+  const float* w2;       /* [3, hidden] */
+} ff_color_mlp_desc;
+
+/* Which rows the colour module evaluates.  Row j (j < count) is sample
+ * r = rows ? rows[j] : j; its features are feat[r * feat_stride + 0 ..
+ * in_feat-1] and its view direction dirs[d * dir_stride + 0..2] with
+ * d = dir_index ? dir_index[r] : r.  count is read on the device when
+ * count_dev is given (capped at n_max), else count = n_max. */
+typedef struct ff_color_rows {
+  const float* feat;
+  int64_t feat_stride;
+  const float* dirs;
+  int64_t dir_stride;
+  const int32_t* dir_index;
+  const int32_t* rows;
+  const int32_t* count_dev;
+  int64_t n_max;
+} ff_color_rows;
+
+typedef struct ff_color_mlp_grads {
+  float* w0;
+  float* b0;
+  float* w1;
+  float* b1;
+  float* w2;
+} ff_color_mlp_grads;
+
+size_t ff_color_mlp_workspace_bytes(const ff_color_mlp_desc* m, int64_t n_max, int backward);
+/* rgb of row j written at rgb[r * rgb_stride + 0..2] (r as above). */
+int ff_color_mlp_fwd(const ff_color_mlp_desc* m, const ff_color_rows* r, float* rgb, int64_t rgb_stride,
+                     void* ws, size_t ws_bytes, void* stream);
+/* Backward: dL/drgb of row j read at d_rgb[r * d_rgb_stride + 0..2]; the
+ * feature gradient of row j WRITTEN (not accumulated) at
+ * d_feat[j * d_feat_stride + 0 .. in_feat-1] (row order j, so a packed
+ * sample list gets a compact block); the weight gradients ACCUMULATED into g
+ * after a fixed-order reduction.  The forward is recomputed. */
+int ff_color_mlp_bwd(const ff_color_mlp_desc* m, const ff_color_rows* r, const float* d_rgb,
+                     int64_t d_rgb_stride, float* d_feat, int64_t d_feat_stride,
+                     const ff_color_mlp_grads* g, void* ws, size_t ws_bytes, void* stream);
+
+/* ============================================================= dropout ==
+ * torch.nn.functional.dropout(x, p) as linear_mat applies it to its input
+ * when with_dropout and is_train (models/FactorFields.py:150-151):
+ * out = x * keep / (1 - p).  keep comes from `keep` [n] (0/1, may be NULL)
+ * or, when NULL, from a counter-based generator (the Squares RNG, Widynski
+ * 2020) keyed by seed at counters offset .. offset + n - 1, kept when its
+ * uniform draw in [0, 1) is below 1 - p.  keep_out [n] (may be NULL) receives
+ * the mask.  The backward is the same call on the output gradient with the
+ * same mask (or seed and offset). */
+int ff_dropout(const float* x, int64_t n, float p, const uint8_t* keep, uint64_t seed, uint64_t offset,
+               float* out, uint8_t* keep_out, void* stream);
 
 /* Occupancy-grid maintenance (models/FactorFields.py:710-793).
  * ff_render_alpha: compute_alpha :710-727 over n points: alpha_accum[i] +=

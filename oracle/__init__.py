@@ -48,12 +48,12 @@ tests/golden/ holds fixtures written from these functions
 
 from .field import (AttrDict, N_to_reso, N_to_vm_reso, OracleField, cal_n_samples,
                     dct_dict, grid_mapping, make_cfg, setup_params)
-from .render import (OracleRenderer, alpha_grid_sample, composite, eval_sh,
-                     raw2alpha, sample_pdf, sample_point)
+from .render import (OracleRenderer, alpha_grid_sample, composite, eval_sh, mlp_render_fea,
+                     positional_encoding, raw2alpha, sample_pdf, sample_point)
 
 __all__ = [
     "AttrDict", "N_to_reso", "N_to_vm_reso", "OracleField", "cal_n_samples",
     "dct_dict", "grid_mapping", "make_cfg", "setup_params", "OracleRenderer",
-    "alpha_grid_sample", "composite", "eval_sh", "raw2alpha", "sample_pdf",
-    "sample_point",
+    "alpha_grid_sample", "composite", "eval_sh", "mlp_render_fea", "positional_encoding",
+    "raw2alpha", "sample_pdf", "sample_point",
 ]

@@ -343,7 +343,15 @@ class OracleField:
         return basises, basises
 
     # -- models/FactorFields.py:144-159 (MLPMixer.forward, pe = 0) ---------
-    def linear_mat(self, h):
+    def linear_mat(self, h, keep=None, p=0.1):
+        """``keep`` (bool, the shape of h) replaces the Bernoulli draw of the
+        input dropout F.dropout(h, p=0.1) (:150-151): kept entries are scaled
+        by 1/(1-p), computed as torch's CPU dropout does (mask / (1-p), then
+        the product)."""
+        if keep is not None:
+            noise = keep.to(h.dtype)
+            noise = noise.div_(1 - p)
+            h = h * noise
         n = len(self.mlp)
         for l, (w, b) in enumerate(self.mlp):
             h = F.linear(h, w, b)

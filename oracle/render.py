@@ -76,6 +76,31 @@ def sh_rgb(viewdirs, feat_sh):
     return torch.sigmoid(eval_sh(2, sh, viewdirs))
 
 
+def positional_encoding(positions, freqs):
+    """models/FactorFields.py:74-79: for each channel c and frequency 2^f the
+    entry c*freqs + f of sin(...), then the same order of cos(...)."""
+    freq_bands = (2 ** torch.arange(freqs).float()).to(positions.device)
+    pts = (positions[..., None] * freq_bands).reshape(positions.shape[:-1] + (freqs * positions.shape[-1],))
+    return torch.cat([torch.sin(pts), torch.cos(pts)], dim=-1)
+
+
+def mlp_render_fea(viewdirs, features, layers, viewpe=6, feape=2):
+    """MLPRender_Fea.forward, models/FactorFields.py:188-203.  ``layers`` is
+    [(weight, bias or None), ...] of torch.nn.Linear layout; ReLU between
+    layers, sigmoid at the end."""
+    indata = [features, viewdirs]
+    if feape > 0:
+        indata += [positional_encoding(features, feape)]
+    if viewpe > 0:
+        indata += [positional_encoding(viewdirs, viewpe)]
+    h = torch.cat(indata, dim=-1)
+    for l, (w, b) in enumerate(layers):
+        h = F.linear(h, w, b)
+        if l != len(layers) - 1:
+            h = F.relu(h)
+    return torch.sigmoid(h)
+
+
 def composite(sigma, rgb, z_vals, dists, distance_scale, white_bg):
     """models/FactorFields.py:878-896 given per-sample sigma/rgb."""
     alpha, weight, bg_weight = raw2alpha(sigma, dists * distance_scale)
@@ -91,14 +116,19 @@ def composite(sigma, rgb, z_vals, dists, distance_scale, white_bg):
 
 class OracleRenderer:
     """FactorFields.forward (models/FactorFields.py:843-898) for the bounded,
-    non-NDC case with the degree-2 SH colour head, over an OracleField."""
+    non-NDC case, over an OracleField.  The colour head is the reference's
+    MLPRender_Fea (:162-203) when ``render_layers`` is given (its
+    [(weight, bias), ...]), else the degree-2 SH head."""
 
-    def __init__(self, field, alpha_volume=None, alpha_aabb=None, step_size=None, n_samples=None):
+    def __init__(self, field, alpha_volume=None, alpha_aabb=None, step_size=None, n_samples=None,
+                 render_layers=None, viewpe=6, feape=2):
         self.field = field
         self.alpha_volume = alpha_volume
         self.alpha_aabb = alpha_aabb
         self.step_size = step_size
         self.n_samples = n_samples
+        self.render_layers = render_layers
+        self.viewpe, self.feape = viewpe, feape
 
     def forward(self, rays, jitter=None, white_bg=True, n_samples=-1):
         f = self.field
@@ -126,7 +156,11 @@ class OracleRenderer:
         ray_valid_new = torch.logical_and(ray_valid, app_mask)
         app_sel = ray_valid_new[ray_valid]
         if app_sel.any():
-            valid_rgbs = sh_rgb(viewdirs[ray_valid_new], feat[app_sel, 1:28])
+            if self.render_layers is not None:
+                valid_rgbs = mlp_render_fea(viewdirs[ray_valid_new], feat[app_sel, 1:], self.render_layers,
+                                            self.viewpe, self.feape)
+            else:
+                valid_rgbs = sh_rgb(viewdirs[ray_valid_new], feat[app_sel, 1:28])
             rgb = rgb.masked_scatter(ray_valid_new[..., None].expand_as(rgb), valid_rgbs)
         acc_map = torch.sum(weight, -1)
         rgb_map = torch.sum(weight[..., None] * rgb, -2)

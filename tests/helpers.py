@@ -83,3 +83,21 @@ def new_model(cfg, dtype="fp32", seed=0):
     m = FactorFields(cfg, "cuda", table_dtype=dtype)
     randomize(m, seed)
     return m
+
+
+def assert_close_fp64(a, b32, b64, rtol, what="", atol_frac=1e-6, slack=2.0):
+    """Element-wise: the kernel result a is within the usual bar of the FP64
+    result b64 (rtol * |b64| + atol_frac * max|b64|), or no further from it
+    than `slack` times the FP32 oracle b32 is.  For sums whose terms cancel,
+    where two FP32 summation orders may differ by more than the bar while
+    both are as close to the exact value as FP32 allows."""
+    a = a.detach().double().cpu()
+    b32 = b32.detach().double().cpu()
+    b64 = b64.detach().double().cpu()
+    assert a.shape == b64.shape == b32.shape, (what, a.shape, b32.shape, b64.shape)
+    scale = b64.abs().max().item() or 1.0
+    err = (a - b64).abs()
+    tol = torch.maximum(rtol * b64.abs() + atol_frac * scale, slack * (b32 - b64).abs())
+    bad = err > tol
+    assert not bad.any(), (f"{what}: {int(bad.sum())}/{bad.numel()} outside tol vs FP64; max abs err "
+                           f"{err.max().item():.3e}, FP32 oracle max abs err {(b32 - b64).abs().max().item():.3e}")

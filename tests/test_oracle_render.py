@@ -173,3 +173,115 @@ def test_sample_pdf_uniform_and_bruteforce():
             t = (uu - cdf[below]) / den
             ref = float(bins[r, below]) + t * (float(bins[r, above]) - float(bins[r, below]))
             assert abs(float(got[r, k]) - ref) < 1e-9
+
+
+# ---------------------------------------------------------------- MLPRender_Fea
+def test_positional_encoding_layout_and_values():
+    """positional_encoding (models/FactorFields.py:74-79): for D channels and F
+    frequencies the output is [sin block | cos block], entry c*F + f of each
+    block at channel c, frequency 2^f.  A single non-zero channel isolates its
+    entries, so a transposed (f*D + c) layout fails."""
+    import math
+    from oracle import positional_encoding
+    D, F = 5, 3
+    for k in range(D):
+        a = 0.37 + 0.1 * k
+        x = torch.zeros((1, D), dtype=torch.float64)
+        x[0, k] = a
+        pe = positional_encoding(x, F)[0]
+        assert pe.shape == (2 * D * F,)
+        for c in range(D):
+            for f in range(F):
+                v = a * 2 ** f if c == k else 0.0
+                assert abs(pe[c * F + f].item() - math.sin(v)) < 1e-15
+                assert abs(pe[D * F + c * F + f].item() - math.cos(v)) < 1e-15
+
+
+def _np_render_fea(viewdirs, feat, layers, viewpe, feape):
+    """MLPRender_Fea.forward (:188-203) in numpy FP64 with the input assembled
+    entry by entry: features, viewdir, sin/cos of features * 2^f, sin/cos of
+    viewdir * 2^f."""
+    import numpy as np
+    def pe(x, F):
+        s = [np.sin(x[:, c] * 2.0 ** f) for c in range(x.shape[1]) for f in range(F)]
+        co = [np.cos(x[:, c] * 2.0 ** f) for c in range(x.shape[1]) for f in range(F)]
+        return np.stack(s + co, 1)
+    h = np.concatenate([feat, viewdirs, pe(feat, feape), pe(viewdirs, viewpe)], 1)
+    for l, (w, b) in enumerate(layers):
+        h = h @ w.T + (0 if b is None else b)
+        if l != len(layers) - 1:
+            h = np.maximum(h, 0)
+    return 1 / (1 + np.exp(-h))
+
+
+def test_mlp_render_fea_against_numpy_and_shapes():
+    import numpy as np
+    from oracle import mlp_render_fea
+    g = torch.Generator().manual_seed(1)
+    N, C, H = 64, 31, 128
+    for viewpe in (6, 2):
+        in_mlpC = 3 + C + 2 * viewpe * 3 + 2 * 2 * C          # models/FactorFields.py:166
+        assert in_mlpC == (194 if viewpe == 6 else 170)
+        layers = [(torch.randn((H, in_mlpC), generator=g, dtype=torch.float64) * 0.1,
+                   torch.randn(H, generator=g, dtype=torch.float64) * 0.1),
+                  (torch.randn((H, H), generator=g, dtype=torch.float64) * 0.1,
+                   torch.randn(H, generator=g, dtype=torch.float64) * 0.1),
+                  (torch.randn((3, H), generator=g, dtype=torch.float64) * 0.1, None)]
+        feat = torch.randn((N, C), generator=g, dtype=torch.float64)
+        d = torch.randn((N, 3), generator=g, dtype=torch.float64)
+        d = d / d.norm(dim=-1, keepdim=True)
+        got = mlp_render_fea(d, feat, layers, viewpe=viewpe, feape=2).numpy()
+        ref = _np_render_fea(d.numpy(), feat.numpy(), [(w.numpy(), None if b is None else b.numpy())
+                                                       for w, b in layers], viewpe, 2)
+        assert np.abs(got - ref).max() < 1e-13
+        assert (got > 0).all() and (got < 1).all()
+
+
+def test_mlp_render_fea_feature_gradient_fp64_central_differences():
+    """The feature gradient through the positional encoding (the chain the
+    colour kernels' backward implements), against FP64 central differences."""
+    from oracle import mlp_render_fea
+    g = torch.Generator().manual_seed(2)
+    N, C, H = 3, 31, 128
+    layers = [(torch.randn((H, 194), generator=g, dtype=torch.float64) * 0.1,
+               torch.randn(H, generator=g, dtype=torch.float64) * 0.1),
+              (torch.randn((H, H), generator=g, dtype=torch.float64) * 0.1,
+               torch.randn(H, generator=g, dtype=torch.float64) * 0.1),
+              (torch.randn((3, H), generator=g, dtype=torch.float64) * 0.1, None)]
+    feat = torch.randn((N, C), generator=g, dtype=torch.float64)
+    d = torch.nn.functional.normalize(torch.randn((N, 3), generator=g, dtype=torch.float64), dim=-1)
+    R = torch.randn((N, 3), generator=g, dtype=torch.float64)
+    f = feat.clone().requires_grad_(True)
+    (mlp_render_fea(d, f, layers) * R).sum().backward()
+    eps = 1e-6
+    for i in range(N):
+        for c in range(0, C, 5):
+            fp, fm = feat.clone(), feat.clone()
+            fp[i, c] += eps
+            fm[i, c] -= eps
+            num = ((mlp_render_fea(d, fp, layers) * R).sum() - (mlp_render_fea(d, fm, layers) * R).sum()) / (2 * eps)
+            assert abs(num.item() - f.grad[i, c].item()) < 1e-7 * max(1.0, abs(num.item()))
+
+
+def test_linear_mat_dropout_with_injected_mask():
+    """OracleField.linear_mat(h, keep): F.dropout(h, p=0.1) with the mask keep
+    is h * keep / 0.9 entering the MLP; numpy FP64 of that closed form."""
+    import numpy as np
+    from oracle import OracleField, make_cfg
+    cfg = make_cfg("sdf", dict(out_dim=3, num_layers=2, hidden_dim=64, coeff_reso=4,
+                               basis_resos=[5, 6, 7, 8, 9, 10], total_params=0),
+                   dict(aabb=[[0.0, 0.0, 0.0], [4.0, 4.0, 4.0]]))
+    of = OracleField(cfg, dtype=torch.float64)
+    g = torch.Generator().manual_seed(3)
+    of.mlp = [(torch.randn((64, 18), generator=g, dtype=torch.float64), torch.randn(64, generator=g,
+                                                                                   dtype=torch.float64)),
+              (torch.randn((3, 64), generator=g, dtype=torch.float64), None)]
+    h = torch.randn((50, 18), generator=g, dtype=torch.float64)
+    keep = torch.rand((50, 18), generator=g) < 0.9
+    got = of.linear_mat(h, keep=keep).numpy()
+    (w0, b0), (w1, _) = [(w.numpy(), None if b is None else b.numpy()) for w, b in of.mlp]
+    x = h.numpy() * keep.numpy() / 0.9
+    ref = np.maximum(x @ w0.T + b0, 0) @ w1.T
+    assert np.abs(got - ref).max() < 1e-12
+    # no mask: the plain decoder
+    assert torch.equal(of.linear_mat(h), of.linear_mat(h, keep=None))
