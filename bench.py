@@ -1,13 +1,19 @@
 """Benchmark: training throughput of the Factor Fields field path on one MI355X.
 
-Default workload (N=1): 3D SDF regression -- 2^20 query points per step
-(half uniform in [-1,1]^3, half near the surface of a seeded union of 32
-spheres and 16 boxes), 3-D grid basis [4,4,4,2,2,2]@[32..128] with sawtooth
-mapping, VM coefficient (three 128^2 planes + three 128 lines, 18 channels),
-MLP 18->64->1, L1 loss, Adam.  One step = batch gather + fused field
-forward/loss/backward kernel + fused Adam over all 11.05 M parameters.
+Default workload (N=1): NeRF-style per-scene training on the synthetic
+Gaussian scene, 4,096 rays per step, in two forms trained the same number of
+steps: the learned proposal with BF16 tables and the matrix-core decoder (the
+line's value, rays/s) and the plain FP32 form with the occupancy sampler
+(timed beside it).  One step = batch gather + sampler + field forward +
+compositing + loss + backward + fused Adam.  The other workloads (3D SDF
+regression, 2D image regression, dictionary image set) run after it at
+bounded sizes and are nested in the same line under config.other_workloads;
+--workload sdf|image|dict runs one of them alone as the line's value.
 
-Prints one JSON line (rank 0).  Only one GPU is used; --gpus N > 1 exits.
+Every leg reports its dominant kernel's roofline fraction and a CPU baseline
+(the oracle, the reference's ATen ops on the host cores, on a bounded sample
+of the same workload).  Prints one JSON line (rank 0).  Only one GPU is used;
+--gpus N > 1 exits.
 """
 
 import argparse
@@ -150,23 +156,23 @@ def nerf_setup(dev, pool_rays=1 << 20, seed_off=0):
     return scene, rays, rgb
 
 
-def nerf_bytes_per_sample(model, table_bytes, stored_coding=False):
-    """Algorithmic HBM bytes per evaluated sample of the field backward:
-    position (12 B) + direction (12 B) + the sample's upstream gradient
-    (sigma, rgb: 16 B) + the sample's coding + read-modify-write of every
-    gathered value's FP32 gradient (8 B each).  The coding is one read of
-    every gathered table value (`table_bytes` each) for the FP32 kernel, or,
-    with stored_coding (BF16 tables), the 2C FP32 basis and coefficient values
-    the forward kept for the sample.  G = 8 corners per basis channel + 8
-    corners per coefficient channel."""
+def nerf_bytes_per_sample(model, table_bytes):
+    """Algorithmic HBM bytes per evaluated sample of the field backward, the
+    per-unit model of the workload definition: position (4 B x 3) + the
+    sample's upstream gradient (sigma, rgb: 4 B x 4) + one read of every
+    gathered table value (`table_bytes` each) + read-modify-write of its FP32
+    gradient (8 B each), with G = 8 corners per basis channel + 8 corners per
+    coefficient channel = 288.  FP32: 3,484 B; BF16: 2,908 B.  (The kernels
+    read the values the forward stored instead of gathering again, so they
+    move fewer bytes than this count credits.)"""
     C = sum(model.basis_dims)
     G = 8 * C + 8 * C
-    coding = 4 * 2 * C if stored_coding else table_bytes * G
-    return 12 + 12 + 16 + coding + 8 * G, G
+    return 12 + 16 + table_bytes * G + 8 * G, G
 
 
-def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pretrain, check_rays=256):
-    """Train one NeRF form and time `timed_steps` steps after `pretrain`.
+def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pretrain, warmup, check_rays=256):
+    """Train one NeRF form for `pretrain` steps (building the occupancy grid
+    halfway), run `warmup` untimed steps, then time `timed_steps` steps.
 
     form 'plain': FP32 tables, uniform sampler with the 128^3 occupancy grid
     (train_per_scene.py:126-200 step order: sample, field, composite, MSE,
@@ -233,6 +239,8 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
             referee_check(it)
     referee_check(pretrain)
     occ_frac = float(model.alphaMask.occ.float().mean()) if model.alphaMask is not None else 1.0
+    for _ in range(warmup):
+        step()
     torch.cuda.synchronize()
     model.kernel_events = []
     t0 = time.perf_counter()
@@ -248,17 +256,19 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
         out = [model.render_eval(test_rays[k:k + 4096], use_proposal=proposal_on(), N_samples=n_samples)
                for k in range(0, test_rays.shape[0], 4096)]
         mse = torch.mean((torch.cat(out) - test_rgb) ** 2).item()
-    bps, G = nerf_bytes_per_sample(model, 2 if prop else 4, stored_coding=prop)
+    bps, G = nerf_bytes_per_sample(model, 2 if prop else 4)
     samples_per_launch = samples / max(timed_steps, 1)
-    return {"rays_per_s": timed_steps * B / dt, "field_pts_per_s": samples / dt,
-            "ms_per_step": dt / timed_steps * 1e3, "samples_per_ray": samples / (timed_steps * B),
-            "test_psnr": -10 * math.log10(mse), "proposal_active": proposal_on(),
-            "fallback_at_step": state["fallback_at"],
-            "sampler_gaps_db": referee.gaps_db[-5:] if referee is not None else [],
-            "train_steps": pretrain + timed_steps, "occupancy_fraction": occ_frac, "n_samples": n_samples,
-            "table_dtype": "bf16" if prop else "fp32", "params": model.n_parameters(),
-            "field_bwd_ms": kern_ms, "field_bwd_bytes_per_sample": bps, "gathered_values_per_sample": G,
-            "field_bwd_gbs": bps * samples_per_launch / (kern_ms * 1e-3) / 1e9 if kern_ms else None}
+    res = {"rays_per_s": timed_steps * B / dt, "field_pts_per_s": samples / dt,
+           "ms_per_step": dt / timed_steps * 1e3, "samples_per_ray": samples / (timed_steps * B),
+           "ns_per_sample": dt / max(samples, 1) * 1e9,
+           "test_psnr": -10 * math.log10(mse), "proposal_active": proposal_on(),
+           "fallback_at_step": state["fallback_at"],
+           "sampler_gaps_db": referee.gaps_db[-5:] if referee is not None else [],
+           "train_steps": pretrain + warmup + timed_steps, "occupancy_fraction": occ_frac, "n_samples": n_samples,
+           "table_dtype": "bf16" if prop else "fp32", "params": model.n_parameters(),
+           "field_bwd_ms": kern_ms, "field_bwd_bytes_per_sample": bps, "gathered_values_per_sample": G,
+           "field_bwd_gbs": bps * samples_per_launch / (kern_ms * 1e-3) / 1e9 if kern_ms else None}
+    return res, model
 
 
 def run_nerf(args):
@@ -273,10 +283,11 @@ def run_nerf(args):
     _, pool, pool_rgb = nerf_setup(dev, pool_rays=args.nerf_pool)
     _, test_rays, test_rgb = nerf_setup(dev, pool_rays=1 << 14, seed_off=7)
     forms = {"both": ["plain", "proposal"], "plain": ["plain"], "proposal": ["proposal"]}[args.nerf_form]
-    res = {}
+    res, models = {}, {}
     for f in forms:
-        log(f"nerf {f}: {args.nerf_pretrain} training steps, then {args.steps} timed")
-        res[f] = nerf_train(f, dev, pool, pool_rgb, test_rays, test_rgb, args.steps, args.nerf_pretrain)
+        log(f"nerf {f}: {args.nerf_pretrain} training steps, {args.warmup} warmup, then {args.steps} timed")
+        res[f], models[f] = nerf_train(f, dev, pool, pool_rgb, test_rays, test_rgb, args.steps, args.nerf_pretrain,
+                                       args.warmup)
         log(f"nerf {f}: {res[f]['rays_per_s']:.0f} rays/s, PSNR {res[f]['test_psnr']:.2f} dB")
     main = res[forms[-1]]
     out = {
@@ -285,29 +296,36 @@ def run_nerf(args):
         "unit": "rays/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.nerf_pretrain,
+        "warmup": args.warmup,
         "ms_per_step": main["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        # proposal form: BF16 table gathers and BF16 matrix-core decoder with
+        # FP32 accumulation; plain form: FP32 throughout
+        "dtype": "bf16" if forms[-1] == "proposal" else "f32",
         "data": "synthetic (48 seeded anisotropic Gaussians, sinusoid colour; ground truth by 1,024-sample "
                 "quadrature on random rays from 100 radius-4 cameras)",
         "config": {"workload": ("nerf_proposal_bf16" if forms[-1] == "proposal" else "nerf_plain_fp32")
                    + ": 4,096 rays/batch, 3-D grid basis [4,4,4,2,2,2]@[32..128], 32^3x18 coefficient grid, "
                      "18-64-28 decoder + degree-2 SH head",
-                   "field_pts_per_s": main["field_pts_per_s"], "forms": res},
+                   "field_pts_per_s": main["field_pts_per_s"], "pretrain_steps": args.nerf_pretrain,
+                   "forms": res},
     }
     if "proposal" in res and "plain" in res:
-        out["config"]["speedup_vs_plain_fp32"] = res["proposal"]["rays_per_s"] / res["plain"]["rays_per_s"]
-        out["config"]["psnr_gap_db"] = res["plain"]["test_psnr"] - res["proposal"]["test_psnr"]
+        pr, pl = res["proposal"], res["plain"]
+        out["config"]["speedup_vs_plain_fp32"] = pr["rays_per_s"] / pl["rays_per_s"]
+        # rays/s ratio = (samples per ray, plain / proposal) x (time per sample, plain / proposal)
+        out["config"]["speedup_split"] = {"samples_per_ray_ratio": pl["samples_per_ray"] / pr["samples_per_ray"],
+                                          "per_sample_time_ratio": pl["ns_per_sample"] / pr["ns_per_sample"]}
+        out["config"]["psnr_gap_db"] = pl["test_psnr"] - pr["test_psnr"]
     bw = main["field_bwd_gbs"]
     kname = "nerf_field_bwd_mfma_kernel" if forms[-1] == "proposal" else "nerf_field_bwd_kernel"
     out["roofline"] = {"bound": "hbm", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": bw / HBM_PEAK_GBS if bw else None, "traffic": measured_traffic(kname),
                        "kernel": kname + " (+ its decoder-gradient reduction)",
                        "kernel_ms": main["field_bwd_ms"], "bytes_per_sample": main["field_bwd_bytes_per_sample"]}
-    return out
+    return out, models.get("plain")
 
 
 def run_dict(args):
@@ -399,7 +417,7 @@ def run_dict(args):
     achieved = bpp * n / (b["train_kernel_ms"] * 1e-3) / 1e9
     out = {"metric": METRIC, "value": b["pts_per_s"], "unit": "field pts/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.dict_steps, "ms_per_step": b["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32",
+           "vs_baseline": None, "dtype": "bf16",
            "data": f"synthetic ({N} seeded Gaussian+sinusoid images, 256x256)",
            "config": {"workload": f"dictionary_image_set_bf16: {N} images, basis [32,32,32,16,16,16]@[32..128], "
                                   "32x32x144 coefficient per image, decoder 144-64-3, 2^22 px/step",
@@ -544,59 +562,187 @@ def cpu_baseline_sdf(args):
                       f"{_cpu_model()}"}
 
 
-def cpu_baseline_nerf(args):
-    """The oracle NeRF path (same ATen ops as the reference: sample_point,
-    grid_sample per level, linear_mat, raw2alpha, SH head, autograd,
-    torch.optim.Adam) on the host cores, plain form without an occupancy
-    grid, on a bounded batch of rays of the same scene."""
+def _oracle_copy(model):
+    """The oracle (CPU, channel-major tables) holding the GPU model's current
+    parameters, for a CPU baseline on the same trained state."""
+    import copy
+    import torch
+    from oracle import OracleField
+    of = OracleField(copy.deepcopy(model.cfg))
+    b, c = model._tables()
+    of.basises = [t.detach().float().cpu().contiguous() for t in b]
+    of.coeffs = [t.detach().float().cpu().contiguous() for t in c]
+    lin = model.linear_mat.backbone
+    of.mlp = [(lin[i].weight.detach().cpu().clone(),
+               None if lin[i].bias is None else lin[i].bias.detach().cpu().clone()) for i in range(len(lin))]
+    return of.requires_grad_(True)
+
+
+def _oracle_adam(of, cfg):
+    import torch
+    return torch.optim.Adam([{"params": [w for w, _ in of.mlp] + [b for _, b in of.mlp if b is not None],
+                              "lr": cfg.training.lr_small},
+                             {"params": of.coeffs + of.basises, "lr": cfg.training.lr_large}],
+                            betas=(0.9, 0.99))
+
+
+def _time_cpu(step, min_steps=3, max_steps=20, budget_s=30.0):
+    """Median wall time of `step` after one untimed call: at least min_steps
+    timed calls, more while under 10 s, never past budget_s."""
+    step()
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < min_steps or (time.perf_counter() - t_start < 10 and len(times) < max_steps):
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > budget_s:
+            break
+    times.sort()
+    return times[len(times) // 2], len(times)
+
+
+def cpu_baseline_nerf(args, model):
+    """The oracle NeRF path (the reference's ATen ops: sample_point,
+    AlphaGridMask grid_sample, grid_sample per level, linear_mat, raw2alpha,
+    SH head, autograd, torch.optim.Adam) on the host cores, on the plain GPU
+    form's trained state: the same tables, decoder, 128^3 occupancy grid and
+    sample count, on a bounded batch of rays of the same scene."""
     import torch
     from ffields import workloads as W
     from ffields.params import N_to_reso, cal_n_samples
-    from oracle import OracleField
     from oracle.render import OracleRenderer
 
     cores = len(os.sched_getaffinity(0))
     torch.set_num_threads(cores)
-    cfg = W.nerf_cfg()
-    of = OracleField(cfg).requires_grad_(True)
-    aabb = of.aabb
-    reso = N_to_reso(cfg.training.volume_resoInit ** 3, aabb)
-    units = (aabb[1] - aabb[0]) / (torch.tensor(reso, dtype=torch.float32) - 1)
-    step = float(units.mean() * cfg.renderer.step_ratio)
+    cfg = model.cfg
+    of = _oracle_copy(model)
+    reso = N_to_reso(cfg.training.volume_resoInit ** 3, of.aabb)
     n_samples = min(int(cfg.renderer.max_samples), cal_n_samples(reso, cfg.renderer.step_ratio))
-    rd = OracleRenderer(of, step_size=step, n_samples=n_samples)
-    opt = torch.optim.Adam([{"params": [w for w, _ in of.mlp] + [b for _, b in of.mlp if b is not None],
-                             "lr": cfg.training.lr_small},
-                            {"params": of.coeffs + of.basises, "lr": cfg.training.lr_large}],
-                           betas=(0.9, 0.99))
+    am = model.alphaMask
+    rd = OracleRenderer(of, None if am is None else am.alpha_volume.cpu()[0, 0],
+                        None if am is None else am.aabb.cpu(), torch.tensor(model._step, dtype=torch.float32),
+                        n_samples)
+    opt = _oracle_adam(of, cfg)
     scene = W.NerfScene(device="cpu")
     g = torch.Generator().manual_seed(W.BASE_SEED + 2000)
     n = args.cpu_rays
     rays = scene.random_rays(n, g)
     rgb = scene.render_gt(rays)
+    evaluated = []
 
     def one():
-        out = rd.forward(rays, white_bg=True, n_samples=n_samples)[0]
+        out, _, aux = rd.forward(rays, jitter=torch.rand(n), white_bg=True, n_samples=n_samples)
+        evaluated.append(int(aux["ray_valid"].sum()))
         loss = torch.mean((out - rgb) ** 2)
         opt.zero_grad()
         loss.backward()
         opt.step()
 
-    one()
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 3 or (time.perf_counter() - t_start < 10 and len(times) < 20):
-        t0 = time.perf_counter()
-        one()
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start > 30:
-            break
-    times.sort()
-    med = times[len(times) // 2]
+    med, k = _time_cpu(one)
+    spr = sum(evaluated) / (len(evaluated) * n)
     return {"value": n / med, "unit": "rays/s", "cores": cores, "kind": "port",
-            "sample": f"{len(times)} timed steps (median) of {n} rays of the same scene and model, plain form "
-                      f"without occupancy grid ({n_samples} samples/ray), oracle (torch CPU, autograd + Adam), "
+            "field_pts_per_s": spr * n / med, "samples_per_ray": spr,
+            "sample": f"{k} timed steps (median) of {n} rays of the same scene on the plain form's trained "
+                      f"state (same tables, decoder and 128^3 occupancy grid, {n_samples} candidates, "
+                      f"{spr:.0f} evaluated samples/ray), oracle (torch CPU, autograd + Adam), {cores} threads "
+                      f"on {_cpu_model()}"}
+
+
+def cpu_baseline_image(args):
+    """The oracle's 2D image regression step (grid_sample per level, MLP,
+    MSE, autograd, Adam) on the host cores, same image, model and batch."""
+    import torch
+    from ffields import workloads as W
+    from oracle import OracleField
+
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    cfg = W.image_cfg()
+    of = OracleField(cfg).requires_grad_(True)
+    opt = _oracle_adam(of, cfg)
+    img = W.synth_image()
+    coords = W.pixel_coords(img.shape[0], img.shape[1])
+    rgb = img.reshape(-1, 3)
+    n = int(cfg.training.batch_size)
+    g = torch.Generator().manual_seed(W.BASE_SEED + 3)
+
+    def one():
+        idx = torch.randint(0, rgb.shape[0], (n,), generator=g)
+        loss = torch.mean((of.field(coords[idx]) - rgb[idx]) ** 2)
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+
+    med, k = _time_cpu(one)
+    return {"value": n / med, "unit": "field pts/s", "cores": cores, "kind": "port",
+            "sample": f"{k} timed steps (median) of {n} pixels of the same image and model, oracle (torch CPU, "
+                      f"grid_sample per level + autograd + Adam), {cores} threads on {_cpu_model()}"}
+
+
+def cpu_baseline_dict(args):
+    """The oracle's dictionary image-set step (3-D coefficient grid_sample
+    over the image stack, 144-channel basis per level, decoder 144-64-3, MSE,
+    autograd, Adam over all 152 M parameters) on the host cores, on a bounded
+    batch of the same workload."""
+    import torch
+    from ffields import workloads as W
+    from oracle import OracleField
+
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    N, S = args.dict_images, 256
+    cfg = W.dict_cfg(N, S)
+    of = OracleField(cfg).requires_grad_(True)
+    opt = _oracle_adam(of, cfg)
+    n = args.cpu_dict_batch
+    g = torch.Generator().manual_seed(W.BASE_SEED + 4001)
+    # targets: the generator's images at the drawn pixels
+    imgs = torch.stack([W.synth_image(S, seed=W.BASE_SEED + 4000 + i) for i in range(8)])
+
+    def one():
+        img = torch.randint(0, 8, (n,), generator=g)
+        p = torch.randint(0, S * S, (n,), generator=g)
+        x = torch.stack([(p % S).float() + 0.5, (p // S).float() + 0.5, img.float() + 0.5], -1)
+        t = imgs[img, p // S, p % S]
+        loss = torch.mean((of.field(x) - t) ** 2)
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+
+    med, k = _time_cpu(one, min_steps=3, max_steps=10)
+    return {"value": n / med, "unit": "field pts/s", "cores": cores, "kind": "port",
+            "sample": f"{k} timed steps (median) of {n} pixels of the {N}-image workload (images 0-7 as "
+                      f"targets), oracle (torch CPU, grid_sample + autograd + Adam over every parameter), "
                       f"{cores} threads on {_cpu_model()}"}
+
+
+def run_others(args, rank):
+    """The SDF, image and dictionary workloads at bounded sizes, each with its
+    roofline and CPU baseline, for nesting in the default line."""
+    import copy
+    import torch
+    out = {}
+    args = copy.copy(args)
+    args.dict_fp32 = args.others_dict_fp32
+    legs = [("sdf", lambda: run_sdf(args)[0], cpu_baseline_sdf),
+            ("image", lambda: run_image(args), cpu_baseline_image),
+            ("dict", lambda: run_dict(args), cpu_baseline_dict)]
+    for name, run, cpu in legs:
+        log(f"other workload: {name}")
+        try:
+            r = run()
+            keep = {k: r[k] for k in ("value", "unit", "ms_per_step", "steps", "warmup", "dtype", "roofline",
+                                      "config") if k in r}
+            if rank == 0 and not args.no_cpu_baseline:
+                log(f"cpu baseline: {name}")
+                keep["cpu_baseline"] = cpu(args)
+            out[name] = keep
+        except Exception as e:          # recorded in the line; the headline stays valid
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"{name} failed: {e}")
+        torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -617,6 +763,11 @@ def main():
     ap.add_argument("--nerf-pretrain", type=int, default=1000)
     ap.add_argument("--cpu-batch", type=int, default=1 << 15)
     ap.add_argument("--cpu-rays", type=int, default=256)
+    ap.add_argument("--cpu-dict-batch", type=int, default=1 << 14)
+    ap.add_argument("--others-dict-fp32", type=int, default=0,
+                    help="nested dictionary leg: also train the FP32 form")
+    ap.add_argument("--others", type=int, default=1,
+                    help="nerf: also run the SDF, image and dictionary workloads (nested in the line)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run three 1 GiB device copies first (reference dispatch for rocprofv3 byte counters)")
@@ -629,17 +780,26 @@ def main():
     if args.copy_calibration:
         copy_calibration()
     if args.workload == "nerf":
-        result = run_nerf(args)
-        if rank == 0 and not args.no_cpu_baseline:
+        result, plain = run_nerf(args)
+        if rank == 0 and not args.no_cpu_baseline and plain is not None:
             log("cpu baseline")
-            result["cpu_baseline"] = cpu_baseline_nerf(args)
+            result["cpu_baseline"] = cpu_baseline_nerf(args, plain)
+        del plain
+        if args.others:
+            result["config"]["other_workloads"] = run_others(args, rank)
         print(json.dumps(result), flush=True)
         return
     if args.workload == "image":
-        print(json.dumps(run_image(args)), flush=True)
+        result = run_image(args)
+        if rank == 0 and not args.no_cpu_baseline:
+            result["cpu_baseline"] = cpu_baseline_image(args)
+        print(json.dumps(result), flush=True)
         return
     if args.workload == "dict":
-        print(json.dumps(run_dict(args)), flush=True)
+        result = run_dict(args)
+        if rank == 0 and not args.no_cpu_baseline:
+            result["cpu_baseline"] = cpu_baseline_dict(args)
+        print(json.dumps(result), flush=True)
         return
     result, _ = run_sdf(args)
     if rank == 0 and not args.no_cpu_baseline:

@@ -158,6 +158,7 @@ __global__ __launch_bounds__(256) void nerf_field_fwd_kernel(ff_field_desc fd, f
     float xv[3] = {s.xyz[i * 3 + 0], s.xyz[i * 3 + 1], s.xyz[i * 3 + 2]};
     float bas[C], cof[C], f[C], h[64], o[O];
     coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
+    if (CK == CK_GRID3 && s.coding) store_coding<C>(s, i, bas, cof);
     combine<C, CK>(fd, bas, cof, f);
     mlp_fwd<C, 64, O, 2>(m, f, h, o);
     const float x0 = o[0] + hd.density_shift;
@@ -204,6 +205,83 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff
     coding_bwd_wave<3, CK, BF, Spec18>(fd, gl, xv, db, dc, active, lds);
   }
   mlp_acc_store<C, 64, O, 2>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
+}
+
+// FP32 tables with stored coding rows, backward in two kernels.  The fused
+// kernel above holds the gathers, the FP32 decoder and the scatter at once:
+// 256 VGPRs + 170 AGPRs, one wave per SIMD, and each phase waits on the one
+// before.  Split:
+//   nerf_decoder_bwd_kernel: the rows the forward stored -> decoder forward
+//     and backward -> the rows' gradients (d basis, d coefficient), written
+//     over the rows; decoder weight-gradient partials as in the fused kernel;
+//   nerf_scatter_kernel: corners and weights from the positions, the row
+//     gradients scattered with the run-merging loop; no decoder registers, so
+//     several waves per SIMD keep the atomic unit busy.
+template <int HEAD>
+__global__ __launch_bounds__(64) void nerf_decoder_bwd_kernel(ff_field_desc fd, ff_mlp_desc m, ff_render_head hd,
+                                                              const float* __restrict__ rays, ff_samples s,
+                                                              int64_t n_rays, const float* __restrict__ d_sample,
+                                                              float* __restrict__ ws) {
+  constexpr int C = Spec18::C;
+  constexpr int O = head_out<HEAD>();
+  using Acc = MlpAcc<C, 64, O, 2>;
+  __shared__ float lds[kWave * 65];
+  Acc acc;
+  acc.zero();
+  const int lane = threadIdx.x;
+  const int64_t n = s.ray_off[n_rays];
+  const int64_t ntiles = (n + kWave - 1) / kWave;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t i = tile * kWave + lane;
+    const bool active = i < n;
+    const int64_t ic = active ? i : n - 1;
+    float bas[C], cof[C], f[C], df[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      bas[k] = s.coding[k * s.cap + ic];
+      cof[k] = s.coding[(C + k) * s.cap + ic];
+    }
+    const int64_t rb = s.ray[ic];
+    const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
+    const float4 gs = reinterpret_cast<const float4*>(d_sample)[ic];
+    combine<C, CK_GRID3>(fd, bas, cof, f);
+    mlp_train_tile_fn<C, 64, O, 2>(
+        m, acc, lds, f, active,
+        [&](const float* o, float* dout) { nerf_dout<HEAD>(hd, o, gs, dx, dy, dz, ic, dout); }, df);
+    float db[C], dc[C];
+    combine_bwd<C, CK_GRID3>(fd, bas, cof, df, db, dc);
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        s.coding[k * s.cap + i] = db[k];
+        s.coding[(C + k) * s.cap + i] = dc[k];
+      }
+    }
+  }
+  mlp_acc_store<C, 64, O, 2>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
+}
+
+__global__ __launch_bounds__(64) void nerf_scatter_kernel(ff_field_desc fd, ff_grads g, ff_samples s,
+                                                          int64_t n_rays) {
+  constexpr int C = Spec18::C;
+  __shared__ float lds[kStageWords];
+  const int lane = threadIdx.x;
+  const int64_t n = s.ray_off[n_rays];
+  const int64_t ntiles = (n + kWave - 1) / kWave;
+  const ff_grads gl = local_grads(g);
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t i = tile * kWave + lane;
+    const bool active = i < n;
+    const int64_t ic = active ? i : n - 1;
+    float xv[3] = {s.xyz[ic * 3 + 0], s.xyz[ic * 3 + 1], s.xyz[ic * 3 + 2]};
+    float db[C], dc[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      db[k] = s.coding[k * s.cap + ic];
+      dc[k] = s.coding[(C + k) * s.cap + ic];
+    }
+    coding_bwd_wave<3, CK_GRID3, false, Spec18, true>(fd, gl, xv, db, dc, active, lds);
+  }
 }
 
 // BF16-table forward with the matrix-core decoder: one wave per 64-sample tile.
@@ -316,6 +394,7 @@ __global__ __launch_bounds__(256) void nerf_alpha_kernel(ff_field_desc fd, ff_ml
                                                          int64_t n, float length,
                                                          float* __restrict__ alpha) {
   constexpr int C = Spec18::C;
+  constexpr int O = head_out<HEAD>();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float xv[3] = {xyz[i * 3 + 0], xyz[i * 3 + 1], xyz[i * 3 + 2]};
@@ -331,7 +410,6 @@ __global__ __launch_bounds__(256) void nerf_alpha_kernel(ff_field_desc fd, ff_ml
     for (int k = 0; k < 8; ++k) acc = acc + (occ.occ[cr.tex[k]] ? cr.w[k] : 0.f);
     if (!(acc > 0.f)) return;   // sigma = 0 -> alpha += 0
   }
-  constexpr int O = head_out<HEAD>();
   float bas[C], cof[C], f[C], h[64], o[O];
   coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
   combine<C, CK>(fd, bas, cof, f);
@@ -406,6 +484,9 @@ constexpr int nerf_partial(int O) { return 64 * 18 + 64 + O * 64; }
 // Backward kernels: one wave per SIMD is resident (registers, LDS), so four
 // 64-thread blocks per CU; each block writes one decoder-gradient partial row.
 constexpr int kBwdBlocksPerCu = 4;
+// The scatter kernel of the split FP32 backward: 64-thread blocks, eight per
+// CU (two waves per SIMD).
+constexpr int kScatterBlocksPerCu = 8;
 
 }  // namespace
 }  // namespace ff
@@ -507,12 +588,20 @@ int render_field_bwd_impl(const ff_field_desc* desc, const ff_mlp_desc* mlp, con
     constexpr int CK = decltype(CKi)::value;
     constexpr bool BF = decltype(BFi)::value;
     constexpr int HEAD = decltype(Hi)::value;
-    if constexpr (BF)
+    if constexpr (BF) {
       hipLaunchKernelGGL((nerf_field_bwd_mfma_kernel<CK, HEAD>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays,
                          sv, n_rays, d_sample, (float*)ws);
-    else
+    } else if (CK == CK_GRID3 && sv.coding) {
+      // stored rows: decoder kernel, then the scatter kernel (reads the rows'
+      // gradients the decoder kernel wrote over them)
+      hipLaunchKernelGGL((nerf_decoder_bwd_kernel<HEAD>), dim3(blocks), dim3(64), 0, st, d, m, h, rays, sv, n_rays,
+                         d_sample, (float*)ws);
+      hipLaunchKernelGGL(nerf_scatter_kernel, dim3(rf_blocks(sv.cap, 64, kScatterBlocksPerCu)), dim3(64), 0, st, d,
+                         g, sv, n_rays);
+    } else {
       hipLaunchKernelGGL((nerf_field_bwd_kernel<CK, BF, HEAD>), dim3(blocks), dim3(64), 0, st, d, m, h, g, rays, sv,
                          n_rays, d_sample, (float*)ws);
+    }
     return rf_status();
   });
   if (r) return r;

@@ -269,12 +269,15 @@ class RenderMixin:
             dev = self.device
             buf = alloc_samples(B, Ns, dev)
             cap = buf["xyz"].shape[0]
-            # with BF16 tables the field forward keeps each sample's basis and
-            # coefficient values here for the backward of the same pass
+            # the field forward keeps each sample's basis and coefficient
+            # values here for the backward of the same pass (BF16 tables, or
+            # FP32 with a grid coefficient, whose backward then runs as a
+            # decoder kernel and a scatter kernel and leaves the rows' gradients
+            # in place of the rows)
+            keep_rows = self.table_dtype == "bf16" or self.coeff_type == "grid"
             buf.update(sigma=torch.empty(cap, device=dev), rgb=torch.empty((cap, 3), device=dev),
                        d_sample=torch.empty((cap, 4), device=dev),
-                       coding=(torch.empty((2 * self._spec.C, cap), device=dev)
-                               if self.table_dtype == "bf16" else None))
+                       coding=(torch.empty((2 * self._spec.C, cap), device=dev) if keep_rows else None))
             self._feature_buffers(buf, B)
             cache[key] = buf
         self._rbuf = (key, cache[key])

@@ -238,12 +238,13 @@ typedef struct ff_samples {
   int64_t cap;
   float* coding;         /* optional [2C, cap], channel-major: the basis values (rows 0..C-1)
                           * and coefficient values (rows C..2C-1) of every sample.  When
-                          * non-NULL and the tables are BF16, ff_render_field_fwd writes them
-                          * and ff_render_field_bwd reads them instead of gathering again, so
-                          * both calls must see the same samples and tables (the autograd
-                          * saved features of the reference's get_coding,
-                          * models/FactorFields.py:523-533).  NULL, or FP32 tables: the
-                          * backward gathers again. */
+                          * non-NULL (BF16 tables, or FP32 tables with a grid coefficient),
+                          * ff_render_field_fwd writes them and ff_render_field_bwd reads them
+                          * instead of gathering again, so both calls must see the same
+                          * samples and tables (the autograd saved features of the
+                          * reference's get_coding, models/FactorFields.py:523-533).  With
+                          * FP32 tables the backward overwrites them with their gradients.
+                          * NULL (or a VM coefficient): the backward gathers again. */
 } ff_samples;
 
 /* rays [n_rays, 6] = (origin, direction); jitter [n_rays] U[0,1) per ray

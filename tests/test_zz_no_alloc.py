@@ -47,6 +47,9 @@ def _one_case(m, n_rays, seed):
             "ff_composite_bwd")
     d, mm = m._field_structs()
     h = m._head()
+    # with FP32 tables the backward overwrites the stored coding rows with
+    # their gradients: give the second call the rows the forward wrote
+    rows = None if buf.get("coding") is None else buf["coding"].clone()
     # the existing entry point, with a caller-owned workspace
     tabs_a, dec_a, ga = _grad_bufs(m)
     nb = L.lib().ff_render_field_bwd_workspace_bytes(ctypes.byref(d), ctypes.byref(mm), s.cap)
@@ -56,6 +59,8 @@ def _one_case(m, n_rays, seed):
                                         ctypes.byref(s), L.ptr(buf["d_sample"]), ctypes.byref(ga), L.ptr(ws), nb,
                                         st), "ff_render_field_bwd")
     # the allocation-free one
+    if rows is not None:
+        buf["coding"].copy_(rows)
     tabs_b, dec_b, gb = _grad_bufs(m)
     L.check(L.lib().ff_render_field_bwd_noalloc(ctypes.byref(d), ctypes.byref(mm), ctypes.byref(h), L.ptr(rays),
                                                 n_rays, ctypes.byref(s), L.ptr(buf["d_sample"]),
