@@ -14,9 +14,10 @@
 // tables the decoder runs on the matrix cores (BF16 operands, FP32
 // accumulation, ff_mfma.h); the FP32 form keeps the FP32 vector decoder.
 #include <mutex>
+#include <type_traits>
 
 #include "ff_field_impl.h"
-#include "ff_mfma.h"
+#include "ff_mfma_f32.h"
 
 namespace ff {
 
@@ -144,30 +145,6 @@ __device__ __forceinline__ void head_out_store(const ff_render_head& hd, const f
 }
 
 template <int CK, bool BF, int HEAD>
-__global__ __launch_bounds__(256) void nerf_field_fwd_kernel(ff_field_desc fd, ff_mlp_desc m,
-                                                             ff_render_head hd,
-                                                             const float* __restrict__ rays,
-                                                             ff_samples s, int64_t n_rays,
-                                                             float* __restrict__ sigma,
-                                                             float* __restrict__ rgb) {
-  constexpr int C = Spec18::C;
-  constexpr int O = head_out<HEAD>();
-  const int64_t n = s.ray_off[n_rays];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    float xv[3] = {s.xyz[i * 3 + 0], s.xyz[i * 3 + 1], s.xyz[i * 3 + 2]};
-    float bas[C], cof[C], f[C], h[64], o[O];
-    coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
-    if (CK == CK_GRID3 && s.coding) store_coding<C>(s, i, bas, cof);
-    combine<C, CK>(fd, bas, cof, f);
-    mlp_fwd<C, 64, O, 2>(m, f, h, o);
-    const float x0 = o[0] + hd.density_shift;
-    sigma[i] = hd.density_act == 0 ? softplusf(x0) : fmaxf(x0, 0.f);
-    head_out_store<HEAD>(hd, o, rays, s, i, rgb);
-  }
-}
-
-template <int CK, bool BF, int HEAD>
 __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff_mlp_desc m,
                                                             ff_render_head hd, ff_grads g,
                                                             const float* __restrict__ rays,
@@ -212,11 +189,14 @@ __global__ __launch_bounds__(64) void nerf_field_bwd_kernel(ff_field_desc fd, ff
 // 256 VGPRs + 170 AGPRs, one wave per SIMD, and each phase waits on the one
 // before.  Split:
 //   nerf_decoder_bwd_kernel: the rows the forward stored -> decoder forward
-//     and backward -> the rows' gradients (d basis, d coefficient), written
-//     over the rows; decoder weight-gradient partials as in the fused kernel;
+//     and backward on the FP32 matrix cores -> the rows' gradients (d basis,
+//     d coefficient), written over the rows; decoder weight-gradient partials
+//     in the fused kernel's row layout;
 //   nerf_scatter_kernel: corners and weights from the positions, the row
 //     gradients scattered with the run-merging loop; no decoder registers, so
 //     several waves per SIMD keep the atomic unit busy.
+// (The BF16 form measured no gain from the same split: 0.339 against
+// 0.325 ms per 196 K-sample batch, so it keeps the fused kernel.)
 template <int HEAD>
 __global__ __launch_bounds__(64) void nerf_decoder_bwd_kernel(ff_field_desc fd, ff_mlp_desc m, ff_render_head hd,
                                                               const float* __restrict__ rays, ff_samples s,
@@ -224,9 +204,12 @@ __global__ __launch_bounds__(64) void nerf_decoder_bwd_kernel(ff_field_desc fd, 
                                                               float* __restrict__ ws) {
   constexpr int C = Spec18::C;
   constexpr int O = head_out<HEAD>();
-  using Acc = MlpAcc<C, 64, O, 2>;
-  __shared__ float lds[kWave * 65];
-  Acc acc;
+  __shared__ __attribute__((aligned(16))) float lds[kF32LdsWords];
+  MfmaDecoderF32<O> W;
+  W.load(m);
+  MfmaDecoderF32Bwd<O> WB;
+  WB.load(m);
+  MfmaDecoderAcc<O> acc;
   acc.zero();
   const int lane = threadIdx.x;
   const int64_t n = s.ray_off[n_rays];
@@ -245,9 +228,14 @@ __global__ __launch_bounds__(64) void nerf_decoder_bwd_kernel(ff_field_desc fd, 
     const float dx = rays[rb * 6 + 3], dy = rays[rb * 6 + 4], dz = rays[rb * 6 + 5];
     const float4 gs = reinterpret_cast<const float4*>(d_sample)[ic];
     combine<C, CK_GRID3>(fd, bas, cof, f);
-    mlp_train_tile_fn<C, 64, O, 2>(
-        m, acc, lds, f, active,
-        [&](const float* o, float* dout) { nerf_dout<HEAD>(hd, o, gs, dx, dy, dz, ic, dout); }, df);
+    mfma_f32_decoder_train(
+        W, WB, f,
+        [&](const float* o, float* dout) {
+          nerf_dout<HEAD>(hd, o, gs, dx, dy, dz, ic, dout);
+#pragma unroll
+          for (int k = 0; k < 32; ++k) dout[k] = (active && k < O) ? dout[k] : 0.f;
+        },
+        df, acc, lds);
     float db[C], dc[C];
     combine_bwd<C, CK_GRID3>(fd, bas, cof, df, db, dc);
     if (active) {
@@ -258,7 +246,7 @@ __global__ __launch_bounds__(64) void nerf_decoder_bwd_kernel(ff_field_desc fd, 
       }
     }
   }
-  mlp_acc_store<C, 64, O, 2>(acc, ws + (int64_t)blockIdx.x * Acc::kPartial);
+  acc.store(ws + (int64_t)blockIdx.x * (kMfH * kMfIn + kMfH + O * kMfH));
 }
 
 __global__ __launch_bounds__(64) void nerf_scatter_kernel(ff_field_desc fd, ff_grads g, ff_samples s,
@@ -284,12 +272,14 @@ __global__ __launch_bounds__(64) void nerf_scatter_kernel(ff_field_desc fd, ff_g
   }
 }
 
-// BF16-table forward with the matrix-core decoder: one wave per 64-sample tile.
-// Bounded to three waves per SIMD (168 VGPRs, a few spills): the gathers are
+// Forward with the matrix-core decoder (BF16 tables: BF16 operands; FP32
+// tables: FP32 operands): one wave per 64-sample tile.  BF16: bounded to
+// three waves per SIMD (168 VGPRs, a few spills): the gathers are
 // latency-bound, and the third wave hides more of them than the spills cost
-// (measured 70 -> 54 us per 196 K samples; four waves spill too much).
-template <int CK, int HEAD>
-__global__ __launch_bounds__(256, 3) void nerf_field_fwd_mfma_kernel(ff_field_desc fd, ff_mlp_desc m,
+// (measured 70 -> 54 us per 196 K samples; four waves spill too much).  FP32:
+// two waves per SIMD (207-231 VGPRs, no spills; three spilled 130-360 B).
+template <int CK, bool BF, int HEAD>
+__global__ __launch_bounds__(256, BF ? 3 : 2) void nerf_field_fwd_mfma_kernel(ff_field_desc fd, ff_mlp_desc m,
                                                                   ff_render_head hd,
                                                                   const float* __restrict__ rays,
                                                                   ff_samples s, int64_t n_rays,
@@ -303,7 +293,7 @@ __global__ __launch_bounds__(256, 3) void nerf_field_fwd_mfma_kernel(ff_field_de
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   if (wave >= ntiles) return;  // the grid is sized for the capacity, not the live count
-  MfmaDecoder<O> W;
+  std::conditional_t<BF, MfmaDecoder<O>, MfmaDecoderF32<O>> W;
   W.load(m);
   for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
     const int64_t i = tile * kWave + lane;
@@ -311,10 +301,13 @@ __global__ __launch_bounds__(256, 3) void nerf_field_fwd_mfma_kernel(ff_field_de
     const int64_t ic = active ? i : n - 1;
     float xv[3] = {s.xyz[ic * 3 + 0], s.xyz[ic * 3 + 1], s.xyz[ic * 3 + 2]};
     float bas[C], cof[C], f[C], o[32];
-    coding_fwd<3, CK, true, Spec18>(fd, xv, bas, cof);
+    coding_fwd<3, CK, BF, Spec18>(fd, xv, bas, cof);
     if (s.coding && active) store_coding<C>(s, i, bas, cof);
     combine<C, CK>(fd, bas, cof, f);
-    mfma_decoder_fwd(W, f, o);
+    if constexpr (BF)
+      mfma_decoder_fwd(W, f, o);
+    else
+      mfma_f32_decoder_fwd(W, f, o);
     if (active) {
       const float x0 = o[0] + hd.density_shift;
       sigma[i] = hd.density_act == 0 ? softplusf(x0) : fmaxf(x0, 0.f);
@@ -513,12 +506,8 @@ int ff_render_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
     constexpr int CK = decltype(CKi)::value;
     constexpr bool BF = decltype(BFi)::value;
     constexpr int HEAD = decltype(Hi)::value;
-    if constexpr (BF)
-      hipLaunchKernelGGL((nerf_field_fwd_mfma_kernel<CK, HEAD>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
-                         n_rays, sigma, rgb);
-    else
-      hipLaunchKernelGGL((nerf_field_fwd_kernel<CK, BF, HEAD>), dim3(blocks), dim3(256), 0, st, d, m, h, rays, sv,
-                         n_rays, sigma, rgb);
+    hipLaunchKernelGGL((nerf_field_fwd_mfma_kernel<CK, BF, HEAD>), dim3(blocks), dim3(256), 0, st, d, m, h, rays,
+                       sv, n_rays, sigma, rgb);
     return rf_status();
   });
 }
