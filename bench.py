@@ -140,7 +140,7 @@ def run_sdf(args):
                    "batch_points": n, "params": model.n_parameters(), "final_l1": final_l1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "field_bwd_kernel (fused forward+loss+backward)",
+                     "kernel": "field_bwd_mfma_kernel (fused forward+loss+backward)",
                      "kernel_ms": kern_ms, "bytes_per_point": bpp, "gathered_values_per_point": G},
     }
     return result, model
@@ -320,10 +320,15 @@ def run_nerf(args):
                                           "per_sample_time_ratio": pl["ns_per_sample"] / pr["ns_per_sample"]}
         out["config"]["psnr_gap_db"] = pl["test_psnr"] - pr["test_psnr"]
     bw = main["field_bwd_gbs"]
-    kname = "nerf_field_bwd_mfma_kernel" if forms[-1] == "proposal" else "nerf_field_bwd_kernel"
+    # the plain form's backward is two kernels (decoder on the stored rows,
+    # then the table scatter); its traffic is the sum of their records
+    knames = (["nerf_field_bwd_mfma_kernel"] if forms[-1] == "proposal"
+              else ["nerf_decoder_bwd_kernel", "nerf_scatter_kernel"])
+    tr = [measured_traffic(k) for k in knames]
     out["roofline"] = {"bound": "hbm", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": bw / HBM_PEAK_GBS if bw else None, "traffic": measured_traffic(kname),
-                       "kernel": kname + " (+ its decoder-gradient reduction)",
+                       "frac": bw / HBM_PEAK_GBS if bw else None,
+                       "traffic": None if None in tr else sum(tr),
+                       "kernel": " + ".join(knames) + " (+ the decoder-gradient reduction)",
                        "kernel_ms": main["field_bwd_ms"], "bytes_per_sample": main["field_bwd_bytes_per_sample"]}
     return out, models.get("plain")
 
@@ -499,7 +504,7 @@ def run_image(args):
                    "psnr_full_image": -10 * math.log10(mse), "params": model.n_parameters()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "field_bwd_kernel (fused forward+loss+backward)",
+                     "kernel": "field_bwd_mfma_kernel (fused forward+loss+backward)",
                      "kernel_ms": kern_ms, "bytes_per_point": bpp, "gathered_values_per_point": G},
     }
 

@@ -18,6 +18,7 @@
 #pragma once
 #include "ff_common.h"
 #include "ff_field_launch.h"
+#include "ff_mfma_f32.h"
 
 namespace ff {
 
@@ -829,6 +830,78 @@ __global__ __launch_bounds__(64) void field_bwd_kernel(ff_field_desc fd, ff_mlp_
   }
 }
 
+// field_bwd_kernel with the decoder on the FP32 matrix cores (18 -> 64 -> O,
+// O <= 32): weights in operand registers, loaded once per wave, instead of
+// 1,200-3,300 weights read through the scalar cache (which the vector decoder
+// spills to lanes as 1,400-3,200 scalar registers).  Same loss modes, same
+// partial-row layout.
+template <int D, int CK, bool BF, class S, int O>
+__global__ __launch_bounds__(64) void field_bwd_mfma_kernel(ff_field_desc fd, ff_mlp_desc m, ff_grads g,
+                                                            const float* __restrict__ x, int64_t n,
+                                                            const float* __restrict__ dsrc, int mode,
+                                                            float loss_scale, float* __restrict__ ws,
+                                                            float* __restrict__ loss_sum) {
+  constexpr int C = S::C;
+  static_assert(C == kMfIn, "the matrix-core decoder takes 18 features");
+  constexpr int XD = XDim<D, CK>::value;
+  __shared__ __attribute__((aligned(16))) float lds[kF32LdsWords > kStageWords ? kF32LdsWords : kStageWords];
+  MfmaDecoderF32<O> W;
+  W.load(m);
+  MfmaDecoderF32Bwd<O> WB;
+  WB.load(m);
+  MfmaDecoderAcc<O> acc;
+  acc.zero();
+  float lsum = 0.f;
+  const int lane = threadIdx.x;
+  const int64_t ntiles = (n + kWave - 1) / kWave;
+  const float grad_scale = (float)(1.0 / ((double)n * O)) * loss_scale;
+  const ff_grads gl = local_grads(g);
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t p = tile * kWave + lane;
+    const bool active = p < n;
+    const int64_t pc = active ? p : n - 1;
+    float xv[XD];
+#pragma unroll
+    for (int a = 0; a < XD; ++a) xv[a] = x[pc * XD + a];
+    float bas[C], cof[C], f[C], df[C];
+    coding_fwd<D, CK, BF, S>(fd, xv, bas, cof);
+    combine<C, CK>(fd, bas, cof, f);
+    // mode 0: d_out read from dsrc; 1: MSE against dsrc; 2: L1 against dsrc
+    mfma_f32_decoder_train(
+        W, WB, f,
+        [&](const float* o, float* dout) {
+#pragma unroll
+          for (int k = 0; k < 32; ++k) {
+            float d = 0.f;
+            if (k < O) {
+              if (mode == 0) {
+                d = dsrc[pc * O + k];
+              } else {
+                const float e = o[k] - dsrc[pc * O + k];
+                if (mode == 1) {
+                  d = 2.f * e * grad_scale;
+                  lsum += active ? e * e : 0.f;
+                } else {
+                  d = (e > 0.f ? 1.f : (e < 0.f ? -1.f : 0.f)) * grad_scale;
+                  lsum += active ? fabsf(e) : 0.f;
+                }
+              }
+            }
+            dout[k] = active ? d : 0.f;
+          }
+        },
+        df, acc, lds);
+    float db[C], dc[C];
+    combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
+    coding_bwd_wave<D, CK, BF, S>(fd, gl, xv, db, dc, active, lds);
+  }
+  acc.store(ws + (int64_t)blockIdx.x * (kMfH * kMfIn + kMfH + O * kMfH));
+  if (mode != 0) {
+    lsum = wave_sum(lsum);
+    if (lane == 0 && loss_sum) unsafeAtomicAdd(loss_sum, lsum);
+  }
+}
+
 // Fast decoder kernels (lane = point) for standalone linear_mat.
 template <int IN, int H, int O, int NL>
 __global__ __launch_bounds__(256) void mlp_fwd_kernel(ff_mlp_desc m, const float* __restrict__ f,
@@ -908,8 +981,12 @@ template <int D, int CK, bool BF, int H, int O, int NL>
 int launch_field_bwd(const ff_field_desc& d, const ff_mlp_desc& m, const ff_grads& g, const float* x,
                      int64_t n, const float* dsrc, int mode, float loss_scale, float* ws,
                      unsigned blocks, float* loss_sum, hipStream_t s) {
-  hipLaunchKernelGGL((field_bwd_kernel<D, CK, BF, Spec18, H, O, NL>), dim3(blocks), dim3(64), 0, s, d,
-                     m, g, x, n, dsrc, mode, loss_scale, ws, loss_sum);
+  if constexpr (NL == 2 && H == kMfH && O <= 32)
+    hipLaunchKernelGGL((field_bwd_mfma_kernel<D, CK, BF, Spec18, O>), dim3(blocks), dim3(64), 0, s, d, m, g, x, n,
+                       dsrc, mode, loss_scale, ws, loss_sum);
+  else
+    hipLaunchKernelGGL((field_bwd_kernel<D, CK, BF, Spec18, H, O, NL>), dim3(blocks), dim3(64), 0, s, d,
+                       m, g, x, n, dsrc, mode, loss_scale, ws, loss_sum);
   return launch_status();
 }
 
