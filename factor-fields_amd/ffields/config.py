@@ -6,6 +6,7 @@ float, as OmegaConf would give them.
 """
 
 import copy
+from collections.abc import Mapping, Sequence
 
 import yaml
 
@@ -27,16 +28,32 @@ class AttrDict(dict):
 
     @staticmethod
     def wrap(d):
-        if isinstance(d, AttrDict):
+        """Nested AttrDicts from any mapping (a dict, an AttrDict, or a
+        mapping-like config such as an OmegaConf DictConfig read from a
+        reference checkpoint) and lists/tuples of them."""
+        if isinstance(d, Mapping):
             return AttrDict({k: AttrDict.wrap(v) for k, v in d.items()})
-        if isinstance(d, dict):
-            return AttrDict({k: AttrDict.wrap(v) for k, v in d.items()})
-        if isinstance(d, list):
+        if isinstance(d, (list, tuple)) or _is_listconfig(d):
             return [AttrDict.wrap(v) for v in d]
         return d
 
     def to_dict(self):
-        return {k: (v.to_dict() if isinstance(v, AttrDict) else v) for k, v in self.items()}
+        return plain_cfg(self)
+
+
+def _is_listconfig(d):
+    # OmegaConf's ListConfig is a Sequence but neither a list nor a str
+    return isinstance(d, Sequence) and not isinstance(d, (str, bytes, list, tuple))
+
+
+def plain_cfg(d):
+    """A config as plain nested dicts and lists (what a checkpoint stores):
+    accepts AttrDicts, dicts and mapping-like configs."""
+    if isinstance(d, Mapping):
+        return {k: plain_cfg(v) for k, v in d.items()}
+    if isinstance(d, (list, tuple)) or _is_listconfig(d):
+        return [plain_cfg(v) for v in d]
+    return d
 
 
 def _parse_scalar(s):

@@ -21,7 +21,7 @@ import torch
 import torch.nn as nn
 
 from . import _lib as L
-from .config import AttrDict
+from .config import AttrDict, plain_cfg
 from .ops import Dropout, Field, FieldSpec, GetCoding, LinearMat, is_texel_major, mlp_desc, texel_major
 from .params import N_to_reso, dct_basis, setup_params, vm_reso
 from .render import AlphaGridMask, RenderMixin
@@ -415,16 +415,42 @@ class FactorFields(RenderMixin, nn.Module):
 
     # ------------------------------------------------ models/FactorFields.py:669-691
     def save(self, path):
-        ckpt = {"state_dict": {k: v.detach().contiguous().cpu() for k, v in self.state_dict().items()},
-                "cfg": self.cfg.to_dict()}
+        """The reference's checkpoint: ``state_dict`` (its keys and shapes
+        only), ``cfg`` as plain nested dicts and lists, and the packed
+        occupancy mask.  A learned proposal's state goes beside it under
+        ``proposal`` (its state_dict and constructor arguments), so the
+        reference's strict load_state_dict reads the file.  The reference's
+        cal_mean_coef (:645-667) averages the coefficients of several scenes;
+        for the one scene here it is the identity and is not applied."""
+        ckpt = {"state_dict": {k: v.detach().contiguous().cpu() for k, v in self.state_dict().items()
+                               if not k.startswith("proposal.")},
+                "cfg": plain_cfg(self.cfg)}
         if self.alphaMask is not None:
             vol = self.alphaMask.alpha_volume.bool().cpu().numpy()
             ckpt["alphaMask.shape"] = vol.shape
             ckpt["alphaMask.mask"] = np.packbits(vol.reshape(-1))
             ckpt["alphaMask.aabb"] = self.alphaMask.aabb.cpu()
+        pr = getattr(self, "proposal", None)
+        if pr is not None:
+            ckpt["proposal"] = {"state_dict": {k: v.detach().contiguous().cpu() for k, v in pr.state_dict().items()},
+                                "args": pr.init_args()}
         torch.save(ckpt, path)
 
     def load(self, ckpt):
+        """Strict, like the reference's load_state_dict (:686): a missing or
+        unexpected key, or a shape that differs, raises before any value is
+        copied.  A ``proposal`` entry (written by save) restores the learned
+        proposal, attaching one when the model has none."""
+        sd = ckpt["state_dict"]
+        own = {k: v for k, v in self.state_dict().items() if not k.startswith("proposal.")}
+        missing, unexpected = sorted(set(own) - set(sd)), sorted(set(sd) - set(own))
+        if missing or unexpected:
+            raise RuntimeError(f"checkpoint state_dict does not match the model: missing {missing}, "
+                               f"unexpected {unexpected}")
+        bad = [k for k in own if tuple(own[k].shape) != tuple(sd[k].shape)]
+        if bad:
+            raise RuntimeError("checkpoint shapes differ: " + ", ".join(
+                f"{k} {tuple(sd[k].shape)} vs {tuple(own[k].shape)}" for k in bad))
         if "alphaMask.aabb" in ckpt:
             length = int(np.prod(ckpt["alphaMask.shape"]))
             vol = torch.from_numpy(np.unpackbits(ckpt["alphaMask.mask"])[:length].reshape(
@@ -432,9 +458,15 @@ class FactorFields(RenderMixin, nn.Module):
             self.alphaMask = AlphaGridMask(self.device, ckpt["alphaMask.aabb"].to(self.device),
                                            vol.float().to(self.device))
         with torch.no_grad():
-            own = self.state_dict()
-            for k, v in ckpt["state_dict"].items():
+            for k, v in sd.items():
                 own[k].copy_(v)
+        if "proposal" in ckpt:
+            pr = getattr(self, "proposal", None)
+            if pr is None:
+                pr = self.enable_proposal(**ckpt["proposal"]["args"])
+            pr.load_state_dict(ckpt["proposal"]["state_dict"], strict=True)
+            if pr.table_dtype == "bf16":
+                pr.refresh_shadow()
         if self.table_dtype == "bf16":
             self.refresh_shadows()
         if "reconstruction" in self.cfg.defaults.mode:

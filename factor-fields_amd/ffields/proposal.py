@@ -39,6 +39,12 @@ class ProposalNet(nn.Module):
         if table_dtype == "bf16":
             self.refresh_shadow()
 
+    def init_args(self):
+        """Constructor arguments that rebuild this network's shape (for
+        checkpoints; the trained values come from its state_dict)."""
+        return {"reso": self.reso, "n_fine": self.n_fine, "density_shift": self.density_shift,
+                "table_dtype": self.table_dtype}
+
     def refresh_shadow(self):
         t = self.grid
         s = torch.empty(t.shape, dtype=torch.bfloat16, device=t.device, memory_format=torch.channels_last_3d)
