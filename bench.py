@@ -189,6 +189,7 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
     from ffields import workloads as W
     from ffields.params import N_to_reso, cal_n_samples
     from ffields.proposal import ProposalReferee
+    from ffields.rays import gather_batch
 
     cfg = W.nerf_cfg()
     torch.manual_seed(W.BASE_SEED)
@@ -212,7 +213,7 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
 
     def step(k=None):
         idx = torch.randint(0, pool.shape[0], (B,), device=dev, generator=gen)
-        rays, rgb = pool[idx], pool_rgb[idx]
+        rays, rgb = gather_batch(pool, pool_rgb, idx)
         if proposal_on():
             referee.compare(model, rays, rgb)          # side stream, overlaps the training pass
             model.render_train_step_proposal(rays, rgb, white_bg=True, reduce=False)

@@ -103,6 +103,11 @@ class ProposalGrads(ctypes.Structure):
                 ("replica_stride", ctypes.c_int64)]
 
 
+class Cameras(ctypes.Structure):
+    _fields_ = [("c2w", c_vp), ("n_cams", c_i32), ("H", c_i32), ("W", c_i32), ("fx", c_f32), ("fy", c_f32),
+                ("cx", c_f32), ("cy", c_f32), ("blender", c_i32), ("normalize", c_i32)]
+
+
 _lib = None
 
 
@@ -154,6 +159,8 @@ def _declare(lib):
         "ff_proposal_bwd_workspace_bytes": (ctypes.c_size_t, [I64]),
         "ff_proposal_bwd": (ctypes.c_int, [P(ProposalDesc), c_vp, c_vp, I64, P(Samples), c_vp, c_f32,
                                            P(ProposalGrads), c_vp, c_vp, ctypes.c_size_t, c_vp]),
+        "ff_camera_rays": (ctypes.c_int, [P(Cameras), c_vp, c_vp, I64, c_vp, c_vp]),
+        "ff_gather_rays": (ctypes.c_int, [c_vp, c_vp, I64, c_vp, I64, c_vp, c_vp, c_vp]),
         "ff_composite_fwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp]),
         "ff_composite_bwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,

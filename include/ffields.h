@@ -477,6 +477,41 @@ int ff_proposal_bwd(const ff_proposal_desc* p, const float* rays, const float* p
                     const ff_proposal_grads* grads, float* loss_sum, void* ws, size_t ws_bytes,
                     void* stream);
 
+/* ------------------------------------------------------------ camera rays
+ * World-space rays of pinhole cameras, one thread per ray: the camera-space
+ * direction through the pixel centre (x + 0.5, y + 0.5),
+ *   ((x + 0.5 - cx) / fx, (y + 0.5 - cy) / fy, 1)      (get_ray_directions,
+ *                                                      dataLoader/ray_utils.py:27-45)
+ *   ((x + 0.5 - cx) / fx, -(y + 0.5 - cy) / fy, -1)    (blender = 1,
+ *                                                      get_ray_directions_blender, :48-66)
+ * optionally divided by its norm (normalize = 1: the blender loader's
+ * directions / torch.norm(directions), dataLoader/blender.py:54-55), then
+ * rotated by c2w[:3, :3] (rays_d = d @ c2w[:3,:3].T) with the origin
+ * c2w[:3, 3] (get_rays, ray_utils.py:69-90).  rays [n, 6] = (o, d).
+ * view/pix: n (camera, row-major pixel) pairs; both NULL: every pixel of
+ * every camera in order (n must be n_cams * H * W), the concatenation the
+ * blender loader builds (blender.py:86-96).  A pair outside the cameras or
+ * the image gives a NaN ray. */
+typedef struct ff_cameras {
+  const float* c2w;   /* [n_cams, 3, 4] row-major camera-to-world */
+  int32_t n_cams;
+  int32_t H, W;
+  float fx, fy;       /* focal lengths in pixels */
+  float cx, cy;       /* principal point (the reference's default: W / 2, H / 2) */
+  int32_t blender;
+  int32_t normalize;
+} ff_cameras;
+int ff_camera_rays(const ff_cameras* cams, const int64_t* view, const int64_t* pix, int64_t n, float* rays,
+                   void* stream);
+
+/* Training batch gather (train_per_scene.py:151-152: allrays[ray_idx],
+ * allrgbs[ray_idx]): out_rays[b] = rays[idx[b]] ([*, 6]) and, when rgbs and
+ * out_rgbs are non-NULL, out_rgbs[b] = rgbs[idx[b]] ([*, 3]), b < B.  Exact
+ * copies.  An index outside [0, n_total) gives NaN rows (nothing is read for
+ * it). */
+int ff_gather_rays(const float* rays, const float* rgbs, int64_t n_total, const int64_t* idx, int64_t B,
+                   float* out_rays, float* out_rgbs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

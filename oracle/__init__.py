@@ -41,6 +41,10 @@ How each function is pinned (checked against something other than itself):
 * proposal              -- explicit-overlap loops for the bound loss, its zero
                            set and FP64 central differences
                            (tests/test_oracle_proposal.py).
+* camera rays           -- closed-form directions of chosen pixels, exact
+                           axis permutations under 90-degree rotations, unit
+                           norms, origins equal to the translation
+                           (tests/test_oracle_rays.py).
 
 tests/golden/ holds fixtures written from these functions
 (tests/golden/make_golden.py); tests/test_golden.py recomputes them.
@@ -48,6 +52,7 @@ tests/golden/ holds fixtures written from these functions
 
 from .field import (AttrDict, N_to_reso, N_to_vm_reso, OracleField, cal_n_samples,
                     dct_dict, grid_mapping, make_cfg, setup_params)
+from .rays import all_rays, gather_batch, get_ray_directions, get_ray_directions_blender, get_rays
 from .render import (OracleRenderer, alpha_grid_sample, composite, eval_sh, mlp_render_fea,
                      positional_encoding, raw2alpha, sample_pdf, sample_point)
 
@@ -55,5 +60,6 @@ __all__ = [
     "AttrDict", "N_to_reso", "N_to_vm_reso", "OracleField", "cal_n_samples",
     "dct_dict", "grid_mapping", "make_cfg", "setup_params", "OracleRenderer",
     "alpha_grid_sample", "composite", "eval_sh", "mlp_render_fea", "positional_encoding",
-    "raw2alpha", "sample_pdf", "sample_point",
+    "raw2alpha", "sample_pdf", "sample_point", "all_rays", "gather_batch", "get_ray_directions",
+    "get_ray_directions_blender", "get_rays",
 ]

@@ -160,3 +160,28 @@ def test_sampler_limits_rejected():
     out.cap = 16 * 2048 - 1                       # capacity one short of n_rays * n_samples
     st = lib.ff_sample_rays(ctypes.byref(s), ctypes.c_void_p(0x2000), None, 16, ctypes.byref(out), None)
     assert st == L.FF_ERR_ARG
+
+
+@fake_pointers
+def test_ray_entry_points_validate_arguments():
+    """ff_camera_rays / ff_gather_rays: empty requests return ok with NULL
+    pointers (an empty tensor has none); inconsistent pointers and sizes are
+    rejected before any launch."""
+    lib = L.lib()
+    P = ctypes.c_void_p
+    c = L.Cameras(P(0x1000), 2, 4, 5, 3.0, 3.0, 2.5, 2.0, 0, 1)
+    assert lib.ff_camera_rays(ctypes.byref(c), None, None, 0, None, None) == L.FF_OK
+    assert lib.ff_camera_rays(ctypes.byref(c), P(0x2000), None, 8, P(0x3000), None) == L.FF_ERR_ARG  # pix missing
+    assert lib.ff_camera_rays(ctypes.byref(c), None, None, 39, P(0x3000), None) == L.FF_ERR_ARG      # != 2*4*5
+    assert lib.ff_camera_rays(ctypes.byref(c), P(0x2000), P(0x2800), 8, None, None) == L.FF_ERR_ARG  # no output
+    assert lib.ff_camera_rays(None, None, None, 0, None, None) == L.FF_ERR_ARG
+    for bad in (lambda x: setattr(x, "n_cams", 0), lambda x: setattr(x, "W", 0), lambda x: setattr(x, "fy", 0.0),
+                lambda x: setattr(x, "c2w", None)):
+        c2 = L.Cameras(P(0x1000), 2, 4, 5, 3.0, 3.0, 2.5, 2.0, 0, 1)
+        bad(c2)
+        assert lib.ff_camera_rays(ctypes.byref(c2), None, None, 40, P(0x3000), None) == L.FF_ERR_ARG
+    assert lib.ff_gather_rays(P(0x1000), P(0x1800), 10, None, 0, None, None, None) == L.FF_OK
+    assert lib.ff_gather_rays(P(0x1000), P(0x1800), 10, P(0x2000), 4, P(0x3000), None, None) == L.FF_ERR_ARG
+    assert lib.ff_gather_rays(P(0x1000), None, 10, None, 4, P(0x3000), None, None) == L.FF_ERR_ARG      # no idx
+    assert lib.ff_gather_rays(P(0x1000), None, -1, P(0x2000), 4, P(0x3000), None, None) == L.FF_ERR_ARG
+    assert lib.ff_gather_rays(P(0x1000), None, 10, P(0x2000), -4, P(0x3000), None, None) == L.FF_ERR_ARG

@@ -17,7 +17,7 @@ import torch
 
 from ffields import FactorFields, N_to_reso, cal_n_samples
 from ffields.config import AttrDict
-from ffields.rays import SimpleSampler
+from ffields.rays import SimpleSampler, gather_batch
 from ffields.render import render_ray
 from ffields.workloads import NerfScene
 
@@ -84,7 +84,7 @@ def test_train_per_scene_loop_with_nerf_yaml(tmp_path):
     reso_mask = None
     for iteration in range(t.n_iters):
         ray_idx = sampler.nextids()
-        rays_train, rgb_train = allrays[ray_idx], allrgbs[ray_idx]
+        rays_train, rgb_train = gather_batch(allrays, allrgbs, ray_idx)
         rgb_map, depth_map, coeffs = render_ray(rays_train, model, chunk=t.batch_size, N_samples=nSamples,
                                                 white_bg=True, ndc_ray=0, device=device, is_train=True)
         assert coeffs.shape[1] == 18
