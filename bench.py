@@ -139,7 +139,7 @@ def run_sdf(args):
                                "sawtooth, VM coeff 3x128^2+3x128 (18 ch), MLP 18-64-1, L1, Adam",
                    "batch_points": n, "params": model.n_parameters(), "final_l1": final_l1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("field_bwd_mfma_kernel (sdf)"),
                      "kernel": "field_bwd_mfma_kernel (fused forward+loss+backward)",
                      "kernel_ms": kern_ms, "bytes_per_point": bpp, "gathered_values_per_point": G},
     }
@@ -504,7 +504,7 @@ def run_image(args):
                                "sawtooth, 64x64x18 coefficient, MLP 18-64-3, MSE, Adam 0.02",
                    "psnr_full_image": -10 * math.log10(mse), "params": model.n_parameters()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("field_bwd_mfma_kernel (image)"),
                      "kernel": "field_bwd_mfma_kernel (fused forward+loss+backward)",
                      "kernel_ms": kern_ms, "bytes_per_point": bpp, "gathered_values_per_point": G},
     }
