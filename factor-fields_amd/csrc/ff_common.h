@@ -18,6 +18,18 @@
 namespace ff {
 
 constexpr int kWave = 64;
+
+// Orders a wave's LDS accesses across a hand-off between its lanes (blocks of
+// one wave): a wave's LDS operations execute in order, so waiting for its
+// outstanding LDS traffic and keeping the compiler from moving memory
+// accesses across is enough.  Unlike a block barrier, whose release fence
+// waits for every outstanding global store and atomic of the wave, it lets
+// the atomics of a scatter drain while the wave computes on.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kHalfPi = 1.57079632679489661923f;
 

@@ -232,16 +232,6 @@ __device__ __forceinline__ void coding_fwd(const ff_field_desc& fd, const float*
 // K and d, from a block of one wave (the staging area belongs to the wave).
 constexpr int kStageWords = 64 * 18 + 64 * 8 * 3;
 
-// Orders the wave's LDS accesses across a hand-off between its lanes: a
-// wave's LDS operations execute in order, so waiting for its outstanding LDS
-// traffic and keeping the compiler from moving memory accesses across is
-// enough.
-__device__ __forceinline__ void wave_lds_fence() {
-  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-  __builtin_amdgcn_wave_barrier();
-  asm volatile("" ::: "memory");
-}
-
 // Two loop forms with the same sums (each run added in point order):
 // - serial: the prefix sum of head counts by shuffles; each pass of heads walks
 //   its runs comparing texels; block barriers around the loop (their release
@@ -893,7 +883,7 @@ __global__ __launch_bounds__(64) void field_bwd_mfma_kernel(ff_field_desc fd, ff
         df, acc, lds);
     float db[C], dc[C];
     combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
-    coding_bwd_wave<D, CK, BF, S>(fd, gl, xv, db, dc, active, lds);
+    coding_bwd_wave<D, CK, BF, S, true>(fd, gl, xv, db, dc, active, lds);
   }
   acc.store(ws + (int64_t)blockIdx.x * (kMfH * kMfIn + kMfH + O * kMfH));
   if (mode != 0) {

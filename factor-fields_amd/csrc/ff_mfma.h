@@ -264,7 +264,9 @@ __device__ __forceinline__ void mfma_decoder_fwd_train(const MfmaDecoder<O>& W, 
 // Backward of mfma_decoder_fwd_train: df [18] per lane from dout [32] (0 for
 // inactive lanes and o >= O): dH^T = W1^T . dout^T, ReLU mask, dX^T =
 // W0a^T . dH^T; then the wave's weight gradients from the LDS operands
-// (K = samples) are added to acc.  Ends with the LDS area free.
+// (K = samples) are added to acc.  Ends with the LDS area free.  For blocks
+// of one wave: the LDS hand-offs are wave fences, so outstanding atomics of
+// an earlier scatter are not waited for here.
 template <int O>
 __device__ __forceinline__ void mfma_decoder_bwd(const MfmaDecoderBwd<O>& WB, const uint32_t (&msk)[2][2],
                                                  const float* dout, float* df, MfmaDecoderAcc<O>& acc,
@@ -309,7 +311,7 @@ __device__ __forceinline__ void mfma_decoder_bwd(const MfmaDecoderBwd<O>& WB, co
   mf_unpack_c(dx, dxs);
 #pragma unroll
   for (int k = 0; k < kMfIn; ++k) df[k] = dxs[k];
-  __syncthreads();
+  wave_lds_fence();
   constexpr int kRow8 = kMfRow / 8;  // bf16x8 units per row
   const bf16x8_t* DT8 = reinterpret_cast<const bf16x8_t*>(DT);
   const bf16x8_t* XT8 = reinterpret_cast<const bf16x8_t*>(XT);
@@ -326,7 +328,7 @@ __device__ __forceinline__ void mfma_decoder_bwd(const MfmaDecoderBwd<O>& WB, co
       acc.gw0[t] = mfma_bf16(GT8[(32 * t + r) * kRow8 + col], b_x, acc.gw0[t]);
     }
   }
-  __syncthreads();
+  wave_lds_fence();
 }
 
 }  // namespace ff

@@ -145,6 +145,7 @@ __device__ __forceinline__ void mfma_f32_decoder_fwd(const MfmaDecoderF32<O>& W,
 // dout) turns the lane's outputs into d out (32 values, zero for o >= O and
 // for inactive lanes); df receives d features [18].  The weight gradients
 // are added to acc.  Uses the wave's kF32LdsWords of LDS and ends with it free.
+// For blocks of one wave (the LDS hand-offs are wave fences).
 template <int O, class DoutFn>
 __device__ __forceinline__ void mfma_f32_decoder_train(const MfmaDecoderF32<O>& W, const MfmaDecoderF32Bwd<O>& WB,
                                                        const float* f, DoutFn&& make_dout, float* df,
@@ -202,7 +203,7 @@ __device__ __forceinline__ void mfma_f32_decoder_train(const MfmaDecoderF32<O>& 
         HT[h * kF32Row + r] = hr[nt][mt][i];
         GT[h * kF32Row + r] = dh[mt][i];
       }
-    __syncthreads();
+    wave_lds_fence();
     // dW1 [o][h] += dout^T . relu(H);  dW0a [h][k] += dH^T . X   (K = samples)
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
@@ -227,7 +228,7 @@ __device__ __forceinline__ void mfma_f32_decoder_train(const MfmaDecoderF32<O>& 
         acc.gw0[t] = mfma_f32(ag[t].w, bx.w, acc.gw0[t]);
       }
     }
-    __syncthreads();
+    wave_lds_fence();
   }
   float dxs[32];
   mf_unpack_c(dx, dxs);
