@@ -22,7 +22,8 @@ import torch.nn as nn
 
 from . import _lib as L
 from .config import AttrDict, plain_cfg
-from .ops import Dropout, Field, FieldSpec, GetCoding, LinearMat, is_texel_major, mlp_desc, texel_major
+from . import ops
+from .ops import FieldSpec, is_texel_major, mlp_desc, texel_major
 from .params import N_to_reso, dct_basis, setup_params, vm_reso
 from .render import AlphaGridMask, RenderMixin
 
@@ -66,8 +67,8 @@ class MLPMixer(nn.Module):
         if self.with_dropout and is_train:
             # a fresh draw per call, seeded from torch's host generator (no device sync)
             seed = int(torch.randint(0, 2 ** 62, (1,)).item()) if keep is None else 0
-            x = Dropout.apply(x, self.dropout_p, seed, keep)
-        return LinearMat.apply(x, *self.params_list())
+            x = ops.dropout(x, self.dropout_p, seed, keep)
+        return ops.linear_mat(x, self.params_list())
 
 
 class FactorFields(RenderMixin, nn.Module):
@@ -251,7 +252,7 @@ class FactorFields(RenderMixin, nn.Module):
     # ------------------------------------------------ models/FactorFields.py:425-533
     def _coding(self, x):
         b, c = self._tables()
-        return GetCoding.apply(self._spec, self._read_tables(), x, *(b + c))
+        return ops.get_coding(self._spec, self._read_tables(), x, b + c)
 
     def get_coeff(self, xyz_sampled):
         return self._coding(xyz_sampled)[1]
@@ -274,7 +275,7 @@ class FactorFields(RenderMixin, nn.Module):
             return self.linear_mat(self.get_coding(x)[0])
         b, c = self._tables()
         p = self.linear_mat.params_list()
-        return Field.apply(self._spec, self._read_tables(), x, len(p), *p, *(b + c))
+        return ops.field(self._spec, self._read_tables(), x, p, b + c)
 
     def basis2density(self, density_features):
         """models/FactorFields.py:639-643."""

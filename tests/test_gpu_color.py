@@ -294,19 +294,19 @@ def test_dropout_injected_mask_matches_oracle():
 def test_dropout_device_rng():
     """Masks drawn on the device: keep rate 1 - p, a fresh mask per call, the
     same mask in the backward, exact scaling by 1 / (1 - p)."""
-    from ffields.ops import Dropout
+    from ffields.ops import dropout
     n = 1 << 20
     x = torch.ones(n, device="cuda", requires_grad=True)
-    y = Dropout.apply(x, 0.1, 12345, None)
+    y = dropout(x, 0.1, 12345, None)
     kept = (y != 0)
     rate = kept.float().mean().item()
     assert abs(rate - 0.9) < 0.002, rate
     assert torch.all(y[kept] == torch.tensor(1.0 / 0.9, dtype=torch.float32).cuda())
     y.backward(torch.ones_like(y) * 2.0)
     assert torch.equal(x.grad != 0, kept)
-    y2 = Dropout.apply(x.detach(), 0.1, 12345, None)
+    y2 = dropout(x.detach(), 0.1, 12345, None)
     assert torch.equal(y2, y.detach())                  # same seed, same mask
-    y3 = Dropout.apply(x.detach(), 0.1, 12346, None)
+    y3 = dropout(x.detach(), 0.1, 12346, None)
     agree = ((y3 != 0) == kept).float().mean().item()
     assert 0.80 < agree < 0.84                          # independent masks agree on 0.9^2 + 0.1^2 = 82 %
 
