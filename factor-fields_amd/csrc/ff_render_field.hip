@@ -412,6 +412,31 @@ __global__ __launch_bounds__(256) void nerf_alpha_kernel(ff_field_desc fd, ff_ml
   alpha[i] += 1.f - expf(-sg * length);
 }
 
+// The SH head on its own (SHRender): rgb = sigmoid(eval_sh(2, sh, dir)) per
+// channel, and its backward d sh_k = d rgb * y (1 - y) * Y_k(dir), the same
+// arithmetic as inside the render kernels.
+__global__ __launch_bounds__(256) void sh_rgb_kernel(const float* __restrict__ dirs, int64_t dir_stride,
+                                                     const float* __restrict__ sh, int64_t sh_stride, int64_t n,
+                                                     const float* __restrict__ d_rgb, float* __restrict__ rgb,
+                                                     float* __restrict__ d_sh) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = dirs[i * dir_stride], y = dirs[i * dir_stride + 1], z = dirs[i * dir_stride + 2];
+  const float* o = sh + i * sh_stride;
+  float Y[9];
+  if (d_rgb) sh2_basis(x, y, z, Y);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float v = sigmoidf(sh2_eval(o + 9 * c, x, y, z));
+    if (rgb) rgb[i * 3 + c] = v;
+    if (d_rgb) {
+      const float dpre = d_rgb[i * 3 + c] * (1.f - v) * v;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) d_sh[i * 27 + 9 * c + k] = dpre * Y[k];
+    }
+  }
+}
+
 namespace {
 
 int rf_status() { return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH; }
@@ -510,6 +535,26 @@ int ff_render_field_fwd(const ff_field_desc* desc, const ff_mlp_desc* mlp, const
                        sv, n_rays, sigma, rgb);
     return rf_status();
   });
+}
+
+int ff_sh_rgb(const float* dirs, int64_t dir_stride, const float* sh, int64_t sh_stride, int64_t n, float* rgb,
+              void* stream) {
+  if (n < 0 || dir_stride < 3 || sh_stride < 27) return FF_ERR_ARG;
+  if (n == 0) return FF_OK;
+  if (!dirs || !sh || !rgb) return FF_ERR_ARG;
+  hipLaunchKernelGGL(sh_rgb_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, dirs,
+                     dir_stride, sh, sh_stride, n, (const float*)nullptr, rgb, (float*)nullptr);
+  return rf_status();
+}
+
+int ff_sh_rgb_bwd(const float* dirs, int64_t dir_stride, const float* sh, int64_t sh_stride, int64_t n,
+                  const float* d_rgb, float* d_sh, void* stream) {
+  if (n < 0 || dir_stride < 3 || sh_stride < 27) return FF_ERR_ARG;
+  if (n == 0) return FF_OK;
+  if (!dirs || !sh || !d_rgb || !d_sh) return FF_ERR_ARG;
+  hipLaunchKernelGGL(sh_rgb_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, dirs,
+                     dir_stride, sh, sh_stride, n, d_rgb, (float*)nullptr, d_sh);
+  return rf_status();
 }
 
 int ff_render_alpha(const ff_field_desc* desc, const ff_mlp_desc* mlp, const ff_render_head* head,

@@ -11,7 +11,9 @@ import os
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libffields.so")
+# FFIELDS_LIB selects another build of the same library (the CPU AddressSanitizer
+# build of the argument checks, tests/asan_build.py); unset, the in-tree one
+LIB_PATH = os.environ.get("FFIELDS_LIB") or os.path.join(_HERE, "libffields.so")
 
 FF_MAX_LEVELS = 8
 
@@ -159,6 +161,8 @@ def _declare(lib):
         "ff_proposal_bwd_workspace_bytes": (ctypes.c_size_t, [I64]),
         "ff_proposal_bwd": (ctypes.c_int, [P(ProposalDesc), c_vp, c_vp, I64, P(Samples), c_vp, c_f32,
                                            P(ProposalGrads), c_vp, c_vp, ctypes.c_size_t, c_vp]),
+        "ff_sh_rgb": (ctypes.c_int, [c_vp, I64, c_vp, I64, I64, c_vp, c_vp]),
+        "ff_sh_rgb_bwd": (ctypes.c_int, [c_vp, I64, c_vp, I64, I64, c_vp, c_vp, c_vp]),
         "ff_camera_rays": (ctypes.c_int, [P(Cameras), c_vp, c_vp, I64, c_vp, c_vp]),
         "ff_gather_rays": (ctypes.c_int, [c_vp, c_vp, I64, c_vp, I64, c_vp, c_vp, c_vp]),
         "ff_composite_fwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,

@@ -250,15 +250,24 @@ class FactorFields(RenderMixin, nn.Module):
         return [(p[0], None)] if len(p) == 1 else [(p[0], p[1]), (p[2], None)]
 
     # ------------------------------------------------ models/FactorFields.py:425-533
-    def _coding(self, x):
+    def _coding(self, x, part=None):
+        """(feats, coeff, basis) of get_coding; part 'coeff' / 'basis' gathers
+        only that part's tables (the other outputs are then that part too)."""
         b, c = self._tables()
-        return ops.get_coding(self._spec, self._read_tables(), x, b + c)
+        rt = self._read_tables()
+        nb = len(b)
+        if part == "coeff" and nb and c:
+            return ops.get_coding(self._spec.part("coeff"), rt[nb:], x, c)
+        if part == "basis" and nb and c and self._spec.x_dim == self._spec.in_dim:
+            # (image-set inputs carry the image index, which only the coefficient reads)
+            return ops.get_coding(self._spec.part("basis"), rt[:nb], x, b)
+        return ops.get_coding(self._spec, rt, x, b + c)
 
     def get_coeff(self, xyz_sampled):
-        return self._coding(xyz_sampled)[1]
+        return self._coding(xyz_sampled, "coeff")[1]
 
     def get_basis(self, x):
-        return self._coding(x)[2]
+        return self._coding(x, "basis")[2]
 
     def get_coding(self, x):
         feats, coeff, _ = self._coding(x)

@@ -64,6 +64,24 @@ class FieldSpec:
         self.n_basis = len(self.basis_dims) if basis_kind == L.BASIS_GRID else 0
         self.n_coeff = {L.COEFF_NONE: 0, L.COEFF_GRID: 1, L.COEFF_VM: 6}[coeff_kind]
 
+    def part(self, which):
+        """The same field with only its basis ('basis': coefficient 'none') or
+        only its coefficient ('coeff': basis 'none'), for get_basis /
+        get_coeff: the kernels then gather only that part's tables."""
+        key = "_part_" + which
+        if getattr(self, key, None) is None:
+            kw = dict(in_dim=self.in_dim, x_dim=self.x_dim, basis_dims=self.basis_dims, basis_reso=self.basis_reso,
+                      mapping=self.mapping, basis_kind=self.basis_kind, basis_interp=self.basis_interp,
+                      coeff_kind=self.coeff_kind, coeff_interp=self.coeff_interp, coeff_ndim=self.coeff_ndim,
+                      coeff_reso_xyz=self.coeff_reso_xyz, map_origin=self.map_origin, map_scale=self.map_scale,
+                      coeff_origin=self.coeff_origin, coeff_inv=self.coeff_inv, table_dtype=self.table_dtype)
+            if which == "basis":
+                kw["coeff_kind"] = L.COEFF_NONE
+            else:
+                kw["basis_kind"] = L.BASIS_NONE
+            setattr(self, key, FieldSpec(**kw))
+        return getattr(self, key)
+
     def desc(self, basis_tabs, coeff_tabs):
         d = L.FieldDesc()
         d.in_dim, d.x_dim = self.in_dim, self.x_dim
@@ -426,6 +444,51 @@ def _do_backward(ctx, d_out):
 
 
 dropout_op.register_autograd(_do_backward, setup_context=_do_setup)
+
+
+# ---- ffields::sh_render: the degree-2 SH colour head (models/sh.py:34-85 + sigmoid)
+@torch.library.custom_op("ffields::sh_render", mutates_args=(), device_types="cuda")
+def sh_render_op(viewdirs: Tensor, sh: Tensor) -> Tensor:
+    d = _f32c(viewdirs).reshape(-1, 3)
+    f = _f32c(sh).reshape(-1, 27)
+    n = d.shape[0]
+    rgb = torch.empty((n, 3), device=d.device, dtype=torch.float32)
+    L.check(L.lib().ff_sh_rgb(L.ptr(d), 3, L.ptr(f), 27, n, L.ptr(rgb), L.stream_of(d)), "ff_sh_rgb")
+    return rgb.reshape(*sh.shape[:-1], 3)
+
+
+@sh_render_op.register_fake
+def _(viewdirs, sh):
+    return sh.new_empty((*sh.shape[:-1], 3), dtype=torch.float32)
+
+
+@torch.library.custom_op("ffields::sh_render_backward", mutates_args=(), device_types="cuda")
+def sh_render_backward_op(viewdirs: Tensor, sh: Tensor, d_rgb: Tensor) -> Tensor:
+    d = _f32c(viewdirs).reshape(-1, 3)
+    f = _f32c(sh).reshape(-1, 27)
+    g = _f32c(d_rgb).reshape(-1, 3)
+    n = d.shape[0]
+    d_sh = torch.empty((n, 27), device=d.device, dtype=torch.float32)
+    L.check(L.lib().ff_sh_rgb_bwd(L.ptr(d), 3, L.ptr(f), 27, n, L.ptr(g), L.ptr(d_sh), L.stream_of(d)),
+            "ff_sh_rgb_bwd")
+    return d_sh.reshape(sh.shape)
+
+
+@sh_render_backward_op.register_fake
+def _(viewdirs, sh, d_rgb):
+    return torch.empty_like(sh, dtype=torch.float32)
+
+
+def _sh_setup(ctx, inputs, output):
+    ctx.save_for_backward(*inputs)
+
+
+def _sh_backward(ctx, d_rgb):
+    viewdirs, sh = ctx.saved_tensors
+    return None, sh_render_backward_op(viewdirs, sh, d_rgb)
+
+
+sh_render_op.register_autograd(_sh_backward, setup_context=_sh_setup)
 
 
 # ---- call forms used by the module

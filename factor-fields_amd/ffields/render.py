@@ -20,6 +20,7 @@ import torch
 import torch.nn as nn
 
 from . import _lib as L
+from . import ops
 from .ops import mlp_desc
 from .params import N_to_reso, cal_n_samples
 
@@ -44,11 +45,16 @@ class AlphaGridMask:
 
 
 class SHRender(nn.Module):
-    """Degree-2 SH colour head (no parameters).  It runs inside the render
-    kernels; it is not callable on its own."""
+    """Degree-2 SH colour head (no parameters): rgb = sigmoid(eval_sh(2,
+    features viewed as [..., 3, 9], viewdirs)) (models/sh.py:34-85 with
+    MLPRender_Fea's output sigmoid, models/FactorFields.py:202).  The render
+    kernels evaluate it inline; called on its own it runs the ffields::sh_render
+    custom op (ff_sh_rgb), differentiable in the features."""
 
     def forward(self, viewdirs, features):
-        raise NotImplementedError("the SH colour head is evaluated inside ff_render_field_fwd")
+        if features.shape[-1] != 27:
+            raise ValueError(f"the SH head takes 27 features (3 x 9), got {features.shape[-1]}")
+        return ops.sh_render_op(viewdirs, features)
 
 
 class MLPRender_Fea(nn.Module):

@@ -477,6 +477,17 @@ int ff_proposal_bwd(const ff_proposal_desc* p, const float* rays, const float* p
                     const ff_proposal_grads* grads, float* loss_sum, void* ws, size_t ws_bytes,
                     void* stream);
 
+/* The degree-2 SH colour head on its own (the render kernels evaluate it
+ * inside ff_render_field_fwd): rgb[i, c] = sigmoid(eval_sh(2, sh[i, 9c..9c+8],
+ * dirs[i])) with models/sh.py:34-85's polynomials (eval_sh, deg 2) and
+ * MLPRender_Fea's output sigmoid (models/FactorFields.py:202).  Rows of dirs
+ * and sh are dir_stride / sh_stride floats apart (>= 3 / >= 27); rgb [n, 3].
+ * ff_sh_rgb_bwd writes d_sh [n, 27] (dense) from d_rgb [n, 3]. */
+int ff_sh_rgb(const float* dirs, int64_t dir_stride, const float* sh, int64_t sh_stride, int64_t n, float* rgb,
+              void* stream);
+int ff_sh_rgb_bwd(const float* dirs, int64_t dir_stride, const float* sh, int64_t sh_stride, int64_t n,
+                  const float* d_rgb, float* d_sh, void* stream);
+
 /* ------------------------------------------------------------ camera rays
  * World-space rays of pinhole cameras, one thread per ray: the camera-space
  * direction through the pixel centre (x + 0.5, y + 0.5),

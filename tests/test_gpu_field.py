@@ -72,6 +72,39 @@ def test_get_coding_backward(name):
         assert_close(t.grad.contiguous(), o.grad, GRAD_RTOL, f"table {k}", atol_frac=1e-5)
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_get_coeff_and_get_basis_alone(name):
+    """get_coeff / get_basis gather only their own tables (a field spec with
+    the other part 'none'); values and gradients equal the oracle's
+    (models/FactorFields.py:425-516), and the other part's tables get no
+    gradient."""
+    m = new_model(_case(name))
+    of = oracle_of(m).requires_grad_(True)
+    x = rand_points(m, 2999, seed=6)
+    g = torch.Generator().manual_seed(10)
+    r1 = torch.randn((2999, m._spec.C), generator=g)
+    r2 = torch.randn((2999, m._spec.C), generator=g)
+    coeff = m.get_coeff(x.cuda())
+    basis = m.get_basis(x.cuda())
+    assert_close(coeff, of.get_coeff(x), FWD_RTOL, "coeff alone")
+    assert_close(basis, of.get_basis(x), FWD_RTOL, "basis alone")
+    m.zero_grad(set_to_none=True)
+    (coeff * r1.cuda()).sum().backward()
+    (of.get_coeff(x) * r1).sum().backward()
+    assert all(t.grad is None for t in m.basises)
+    for k, (t, o) in enumerate(zip(m.coeffs, of.coeffs)):
+        assert_close(t.grad.contiguous(), o.grad, GRAD_RTOL, f"coeff table {k}", atol_frac=1e-5)
+    m.zero_grad(set_to_none=True)
+    for t in of.basises:
+        t.grad = None
+    (basis * r2.cuda()).sum().backward()
+    (of.get_basis(x) * r2).sum().backward()
+    # image-set inputs (x carries the image index) take the full coding kernel: zero coefficient gradients
+    assert all(t.grad is None or not t.grad.any() for t in m.coeffs)
+    for k, (t, o) in enumerate(zip(m.basises, of.basises)):
+        assert_close(t.grad.contiguous(), o.grad, GRAD_RTOL, f"basis table {k}", atol_frac=1e-5)
+
+
 @pytest.mark.parametrize("layers,out_dim", [(2, 3), (2, 1), (2, 28), (1, 1), (2, 5)])
 def test_linear_mat(layers, out_dim):
     m = new_model(small_cfg(mode="sdf", D=3, out_dim=out_dim, layers=layers))

@@ -192,3 +192,27 @@ def test_empty_rays_and_misses():
     rgb, depth, _ = m(rays.cuda(), white_bg=True, is_train=False)
     assert torch.allclose(rgb, torch.ones_like(rgb))
     assert torch.allclose(depth, torch.zeros_like(depth))
+
+
+def test_sh_render_module_matches_oracle():
+    """SHRender on its own (ff_sh_rgb / ff_sh_rgb_bwd through the
+    ffields::sh_render custom op) against the oracle's sh_rgb
+    (models/sh.py:34-85 + sigmoid): rgb to 1e-6, feature gradients to 1e-5."""
+    print("[case] SHRender", file=sys.stderr, flush=True)
+    from ffields.render import SHRender
+    g = torch.Generator().manual_seed(31)
+    d = torch.randn((3000, 3), generator=g)
+    d = d / d.norm(dim=-1, keepdim=True)
+    f = torch.randn((3000, 27), generator=g)
+    r = torch.rand((3000, 3), generator=g)
+    fc = f.cuda().requires_grad_(True)
+    out = SHRender()(d.cuda(), fc)
+    (out * r.cuda()).sum().backward()
+    fr = f.clone().requires_grad_(True)
+    ref = sh_rgb(d, fr)
+    (ref * r).sum().backward()
+    assert_close(out, ref, 1e-6, "SH rgb")
+    assert_close(fc.grad, fr.grad, 1e-5, "SH feature grad", atol_frac=1e-6)
+    # batched leading dims
+    out2 = SHRender()(d.cuda().reshape(30, 100, 3), fc.detach().reshape(30, 100, 27))
+    assert torch.equal(out2.reshape(-1, 3), out.detach())
