@@ -170,7 +170,8 @@ def nerf_bytes_per_sample(model, table_bytes):
     return 12 + 16 + table_bytes * G + 8 * G, G
 
 
-def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pretrain, warmup, check_rays=256):
+def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pretrain, warmup, check_rays=256,
+               heldout=None):
     """Train one NeRF form for `pretrain` steps (building the occupancy grid
     halfway), run `warmup` untimed steps, then time `timed_steps` steps.
 
@@ -257,6 +258,11 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
         out = [model.render_eval(test_rays[k:k + 4096], use_proposal=proposal_on(), N_samples=n_samples)
                for k in range(0, test_rays.shape[0], 4096)]
         mse = torch.mean((torch.cat(out) - test_rgb) ** 2).item()
+        heldout_psnr = []
+        for hr, hg in (heldout or []):
+            img = torch.cat([model.render_eval(hr[k:k + 16384], use_proposal=proposal_on(), N_samples=n_samples)
+                             for k in range(0, hr.shape[0], 16384)])
+            heldout_psnr.append(-10 * math.log10(torch.mean((img - hg) ** 2).item()))
     bps, G = nerf_bytes_per_sample(model, 2 if prop else 4)
     samples_per_launch = samples / max(timed_steps, 1)
     res = {"rays_per_s": timed_steps * B / dt, "field_pts_per_s": samples / dt,
@@ -265,8 +271,14 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
            "test_psnr": -10 * math.log10(mse), "proposal_active": proposal_on(),
            "fallback_at_step": state["fallback_at"],
            "sampler_gaps_db": referee.gaps_db[-5:] if referee is not None else [],
+           "sampler_gap_db_stats": ({"checks": len(referee.gaps_db), "min": min(referee.gaps_db),
+                                     "max": max(referee.gaps_db),
+                                     "mean": sum(referee.gaps_db) / len(referee.gaps_db)}
+                                    if referee is not None and referee.gaps_db else None),
            "train_steps": pretrain + warmup + timed_steps, "occupancy_fraction": occ_frac, "n_samples": n_samples,
            "table_dtype": "bf16" if prop else "fp32", "params": model.n_parameters(),
+           "heldout_views_psnr": heldout_psnr or None,
+           "heldout_psnr_mean": sum(heldout_psnr) / len(heldout_psnr) if heldout_psnr else None,
            "field_bwd_ms": kern_ms, "field_bwd_bytes_per_sample": bps, "gathered_values_per_sample": G,
            "field_bwd_gbs": bps * samples_per_launch / (kern_ms * 1e-3) / 1e9 if kern_ms else None}
     return res, model
@@ -283,12 +295,23 @@ def run_nerf(args):
     log("nerf: ground-truth ray pool")
     _, pool, pool_rgb = nerf_setup(dev, pool_rays=args.nerf_pool)
     _, test_rays, test_rgb = nerf_setup(dev, pool_rays=1 << 14, seed_off=7)
+    heldout = None
+    if args.nerf_test_views > 0:
+        # whole images from cameras that are not among the 100 training views
+        from ffields import workloads as W
+        res_px = args.nerf_test_res
+        hs = W.NerfScene(n_views=args.nerf_test_views, H=res_px, W=res_px, device=dev)
+        log(f"nerf: {args.nerf_test_views} held-out views at {res_px}x{res_px}")
+        heldout = []
+        for v in range(args.nerf_test_views):
+            r = hs.rays(v, torch.arange(res_px * res_px, device=dev))
+            heldout.append((r, hs.render_gt(r)))
     forms = {"both": ["plain", "proposal"], "plain": ["plain"], "proposal": ["proposal"]}[args.nerf_form]
     res, models = {}, {}
     for f in forms:
         log(f"nerf {f}: {args.nerf_pretrain} training steps, {args.warmup} warmup, then {args.steps} timed")
         res[f], models[f] = nerf_train(f, dev, pool, pool_rgb, test_rays, test_rgb, args.steps, args.nerf_pretrain,
-                                       args.warmup)
+                                       args.warmup, heldout=heldout)
         log(f"nerf {f}: {res[f]['rays_per_s']:.0f} rays/s, PSNR {res[f]['test_psnr']:.2f} dB")
     main = res[forms[-1]]
     out = {
@@ -767,6 +790,9 @@ def main():
     ap.add_argument("--nerf-pool", type=int, default=1 << 20, help="ground-truth rays the batches are drawn from")
     ap.add_argument("--image-iters", type=int, default=2000)
     ap.add_argument("--nerf-pretrain", type=int, default=1000)
+    ap.add_argument("--nerf-test-views", type=int, default=0,
+                    help="also score whole held-out views (cameras outside the 100 training views)")
+    ap.add_argument("--nerf-test-res", type=int, default=800)
     ap.add_argument("--cpu-batch", type=int, default=1 << 15)
     ap.add_argument("--cpu-rays", type=int, default=256)
     ap.add_argument("--cpu-dict-batch", type=int, default=1 << 14)
