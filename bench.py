@@ -146,6 +146,44 @@ def run_sdf(args):
     return result, model
 
 
+def run_atomics(args):
+    """The FP32 atomic line-request rate (ff_atomic_rate_probe): 2^26 line
+    requests into a 64 MiB target at 4, 16 and 64 lines per wave
+    instruction, timed with HIP events on the launch stream.  The ceiling the
+    backward kernels' table scatters are compared with."""
+    import ctypes
+    import torch
+    import ffields._lib as L
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    n_lines = 1 << 20
+    dst = torch.zeros(n_lines * 16, device=dev)
+    total = 1 << 26
+    rates = {}
+    st = torch.cuda.current_stream()
+    for lpi in (4, 16, 64):
+        n_instr = total // lpi
+        for _ in range(max(args.warmup, 1)):
+            L.check(L.lib().ff_atomic_rate_probe(L.ptr(dst), n_lines, n_instr, lpi, L.stream_of(dst)), "probe")
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        for _ in range(args.steps):
+            L.check(L.lib().ff_atomic_rate_probe(L.ptr(dst), n_lines, n_instr, lpi, L.stream_of(dst)), "probe")
+        b.record(st)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / args.steps
+        rates[lpi] = {"ms_per_launch": ms, "g_line_requests_per_s": total / (ms * 1e-3) / 1e9}
+        log(f"atomics: {lpi} lines/instruction: {rates[lpi]['g_line_requests_per_s']:.2f} G line requests/s")
+    best = max(r["g_line_requests_per_s"] for r in rates.values())
+    return {"metric": METRIC, "value": best, "unit": "G atomic line requests/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": rates[64]["ms_per_launch"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (hashed line indices)",
+            "config": {"workload": "fp32 atomic line-request rate: 2^26 requests into 64 MiB", "rates": rates},
+            "roofline": {"bound": "atomics", "achieved": best, "peak": None, "unit": "G line requests/s",
+                         "frac": None, "traffic": None, "kernel": "atomic_rate_kernel"},
+            "cpu_baseline": None}, None
+
+
 def nerf_setup(dev, pool_rays=1 << 20, seed_off=0):
     import torch
     from ffields import workloads as W
@@ -779,7 +817,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="nerf", choices=["sdf", "nerf", "image", "dict"])
+    ap.add_argument("--workload", default="nerf", choices=["sdf", "nerf", "image", "dict", "atomics"])
     ap.add_argument("--dict-images", type=int, default=1024)
     ap.add_argument("--dict-batch", type=int, default=1 << 22)
     ap.add_argument("--dict-steps", type=int, default=100, help="training steps before the evaluation")
@@ -819,6 +857,10 @@ def main():
         del plain
         if args.others:
             result["config"]["other_workloads"] = run_others(args, rank)
+        print(json.dumps(result), flush=True)
+        return
+    if args.workload == "atomics":
+        result, _ = run_atomics(args)
         print(json.dumps(result), flush=True)
         return
     if args.workload == "image":

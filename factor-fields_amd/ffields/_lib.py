@@ -163,6 +163,7 @@ def _declare(lib):
                                            P(ProposalGrads), c_vp, c_vp, ctypes.c_size_t, c_vp]),
         "ff_sh_rgb": (ctypes.c_int, [c_vp, I64, c_vp, I64, I64, c_vp, c_vp]),
         "ff_sh_rgb_bwd": (ctypes.c_int, [c_vp, I64, c_vp, I64, I64, c_vp, c_vp, c_vp]),
+        "ff_atomic_rate_probe": (ctypes.c_int, [c_vp, I64, I64, c_i32, c_vp]),
         "ff_camera_rays": (ctypes.c_int, [P(Cameras), c_vp, c_vp, I64, c_vp, c_vp]),
         "ff_gather_rays": (ctypes.c_int, [c_vp, c_vp, I64, c_vp, I64, c_vp, c_vp, c_vp]),
         "ff_composite_fwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,

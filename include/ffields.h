@@ -523,6 +523,14 @@ int ff_camera_rays(const ff_cameras* cams, const int64_t* view, const int64_t* p
 int ff_gather_rays(const float* rays, const float* rgbs, int64_t n_total, const int64_t* idx, int64_t B,
                    float* out_rays, float* out_rgbs, void* stream);
 
+/* Measurement only (no counterpart in the reference): the FP32 atomic
+ * line-request rate that bounds every backward's table scatter.  Issues
+ * n_instr wave instructions of global_atomic_add_f32 (+1.0), each touching
+ * lines_per_instr (4, 16 or 64) distinct 64-byte lines of dst
+ * ([n_lines * 16] floats), lines spread by a hash.  Time it with events on
+ * `stream`; n_instr * lines_per_instr line requests. */
+int ff_atomic_rate_probe(float* dst, int64_t n_lines, int64_t n_instr, int32_t lines_per_instr, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
