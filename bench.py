@@ -403,8 +403,9 @@ def run_dict(args):
     --dict-order cell (default) each batch arrives sorted by image and
     coefficient cell (same distribution, drawn inside the timed step), so the
     fused kernel merges the atomics of pixels that share texels.  The BF16
-    form (fused matrix-core kernel) is timed; both forms are trained for the
-    same steps and their PSNR on 16 whole images is reported."""
+    form is the line's value; the FP32 form (FP32 tables, FP32 matrix-core
+    decoder, the same fused kernel layout) is trained for the same steps and
+    timed beside it, and both forms' PSNR on 16 whole images is reported."""
     import torch
     from ffields import FactorFields, FusedAdam
     from ffields import workloads as W
@@ -456,7 +457,7 @@ def run_dict(args):
                 torch.cuda.synchronize()
                 log(f"dict {form}: step {it + 1}, {(time.perf_counter() - t_tr) / (it + 1) * 1e3:.1f} ms/step")
         torch.cuda.synchronize()
-        timed = args.steps if form == "bf16" else 0
+        timed = args.steps if (form == "bf16" or fused) else 0
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(timed)]
         t0 = time.perf_counter()
         for k in range(timed):

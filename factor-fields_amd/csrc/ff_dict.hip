@@ -1,8 +1,19 @@
 // Dictionary image-set field: image-set mode (models/FactorFields.py:286-288,
 // :469-470) with a shared 2-D basis of 144 channels (basis_dims
 // [32,32,32,16,16,16]), a per-image 3-D coefficient grid [n_images, h, w, 144]
-// (get_coeff, :425-465) and a 144 -> 64 -> O decoder (MLPMixer, :113-159), BF16
-// tables with FP32 accumulation.
+// (get_coeff, :425-465) and a 144 -> 64 -> O decoder (MLPMixer, :113-159), in
+// two forms: BF16 tables and BF16 decoder operands with FP32 accumulation
+// (BF = true), or FP32 tables and FP32 decoder operands (BF = false).
+//
+// The FP32 form runs the same products on the FP32 matrix cores: each BF16
+// instruction, whose lanes hold 8 consecutive K values, becomes 8 FP32
+// instructions (v_mfma_f32_16x16x4_f32 for v_mfma_f32_16x16x32_bf16,
+// v_mfma_f32_32x32x2_f32 for v_mfma_f32_32x32x16_bf16) that take element j of
+// every lane's 8 values in step j -- the same operand registers, 8 floats
+// instead of 8 bf16 per lane.  Its layer-1 and data-gradient operands (77 KB
+// in FP32) do not fit in LDS beside the per-wave areas, so the backward reads
+// them from a packed copy in the workspace (L2-resident) and the forward
+// keeps only the layer-1 operands in LDS.
 //
 // Decomposition: a wave handles 16 pixels with 4 lanes per pixel; lane
 // (pixel p, group g) owns the 4-channel pieces 16·mt + 4g .. +3 (mt = 0..8).
@@ -21,14 +32,33 @@
 // and scattered so that every atomic instruction carries four full 64-byte
 // lines (16 lanes on 16 consecutive channels of one texel); consecutive pixels
 // on the same texel and corner are summed first and share one atomic.
+#include <type_traits>
+
 #include "ff_field_impl.h"
 
 namespace ff {
 namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 dbf8;
+typedef __attribute__((__vector_size__(8 * sizeof(float)))) float df8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float df4;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float df16;
+
+// Operand element and vector types of the two forms.
+template <bool BF> struct DType;
+template <> struct DType<true> {
+  using E = __bf16;
+  using V = dbf8;
+  static constexpr int kRow = 24;  // LDS row of 16 pixels + 8 padding (16-byte aligned reads)
+};
+template <> struct DType<false> {
+  using E = float;
+  using V = df8;
+  static constexpr int kRow = 20;  // 16 pixels + 4 padding
+};
+template <bool BF> using DE = typename DType<BF>::E;
+template <bool BF> using DV = typename DType<BF>::V;
+template <bool BF> __device__ __forceinline__ DE<BF> dcvt(float f) { return (DE<BF>)f; }
 
 constexpr int kDC = 144;  // channels
 constexpr int kDH = 64;   // hidden units
@@ -46,6 +76,18 @@ __device__ __forceinline__ df4 mfma16(const dbf8& a, const dbf8& b, const df4& c
 }
 __device__ __forceinline__ df16 mfma32(const dbf8& a, const dbf8& b, const df16& c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+// FP32: lane element j is K index (lane / 16) of step j (16x16x4) or (lane /
+// 32) of step j (32x32x2); over the 8 steps the K order is the BF16 one.
+__device__ __forceinline__ df4 mfma16(const df8& a, const df8& b, df4 c) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j], c, 0, 0, 0);
+  return c;
+}
+__device__ __forceinline__ df16 mfma32(const df8& a, const df8& b, df16 c) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) c = __builtin_amdgcn_mfma_f32_32x32x2f32(a[j], b[j], c, 0, 0, 0);
+  return c;
 }
 
 // Orders this wave's LDS accesses across a hand-off between lanes: the wave's
@@ -76,25 +118,59 @@ struct DictW {
   dbf8 a1[4][5][64];
   dbf8 a4[kDMt][2][64];
 };
+// FP32 forward: layer-1 operands only (the data gradient is not needed).
+struct DictW1F {
+  df8 a1[4][5][64];
+};
+constexpr int kDOpA1 = 4 * 5 * 64, kDOpA4 = kDMt * 2 * 64;  // operand vectors per matrix
+constexpr size_t kDPackBytes = (size_t)(kDOpA1 + kDOpA4) * sizeof(df8);
+
+// Operand sources: a1(mt, ks, lane), a4(mt, ks, lane).
+struct DictWLds {  // BF16, block-shared LDS
+  const DictW& w;
+  __device__ __forceinline__ dbf8 a1(int mt, int ks, int l) const { return w.a1[mt][ks][l]; }
+  __device__ __forceinline__ dbf8 a4(int mt, int ks, int l) const { return w.a4[mt][ks][l]; }
+};
+struct DictW1Lds {  // FP32 forward: layer 1 from LDS
+  const DictW1F& w;
+  __device__ __forceinline__ df8 a1(int mt, int ks, int l) const { return w.a1[mt][ks][l]; }
+};
+struct DictWPack {  // FP32 backward: the packed copy in the workspace
+  const df8* __restrict__ p;
+  __device__ __forceinline__ df8 a1(int mt, int ks, int l) const { return p[(mt * 5 + ks) * 64 + l]; }
+  __device__ __forceinline__ df8 a4(int mt, int ks, int l) const { return p[kDOpA1 + (mt * 2 + ks) * 64 + l]; }
+};
 
 // Per-wave LDS area: weight-gradient operands (phase A, bf16 rows of 16 pixels
 // + 8 padding), then, aliased, the scatter staging (phase B).
-constexpr int kDRow = 24;
+template <bool BF>
 struct DictPhaseA {
-  __bf16 xt[160][kDRow];
-  __bf16 gt[64][kDRow];
-  __bf16 ht[64][kDRow];
-  __bf16 dt[32][kDRow];
+  static constexpr int R = DType<BF>::kRow;
+  DE<BF> xt[160][R];
+  DE<BF> gt[64][R];
+  DE<BF> ht[64][R];
+  DE<BF> dt[32][R];
 };
 constexpr int kDVal = kDC + 1;  // staging row (odd stride: conflict-free column writes)
 struct DictPhaseB {
   float db[kDPx][kDVal];
   float dc[kDPx][kDVal];
 };
+template <bool BF>
 union DictWave {
-  DictPhaseA a;
+  DictPhaseA<BF> a;
   DictPhaseB b;
 };
+
+// Eight consecutive operand values (pixels) of an LDS row.
+__device__ __forceinline__ dbf8 lds_v8(const __bf16* p) { return *reinterpret_cast<const dbf8*>(p); }
+__device__ __forceinline__ df8 lds_v8(const float* p) {
+  const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+  df8 v;
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  return v;
+}
 // Corners of the wave's pixels, written during the gather and read by the
 // scatter (not aliased with the areas above).
 struct __attribute__((aligned(16))) DictCorners {
@@ -129,24 +205,43 @@ __device__ __forceinline__ int dict_runs(int t, float w, int tp, float wp, int k
   return __popcll(H);
 }
 
-__device__ __forceinline__ void load_dict_weights(const ff_mlp_desc& m, DictW& W) {
-  for (int e = threadIdx.x; e < 4 * 5 * 64 + kDMt * 2 * 64; e += blockDim.x) {
-    dbf8 v;
-    if (e < 4 * 5 * 64) {
-      const int l = e & 63, ks = (e >> 6) % 5, mt = e / 320;
-      const int r = l & 15, g = l >> 4;
+// Operand vector e of [a1 (kDOpA1) | a4 (kDOpA4)], in a1's / a4's
+// [mt][ks][lane] order: layer 1's A (W0a rows 16 mt + r, K slots of step ks)
+// and the data gradient's A (W0a^T).
+template <class V>
+__device__ __forceinline__ V dict_operand(const ff_mlp_desc& m, int e) {
+  V v;
+  if (e < kDOpA1) {
+    const int l = e & 63, ks = (e >> 6) % 5, mt = e / 320;
+    const int r = l & 15, g = l >> 4;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = (__bf16)w0a(m, 16 * mt + r, dperm(ks, g, j));
-      W.a1[mt][ks][l] = v;
-    } else {
-      const int e2 = e - 4 * 5 * 64;
-      const int l = e2 & 63, ks = (e2 >> 6) & 1, mt = e2 >> 7;
-      const int r = l & 15, g = l >> 4;
+    for (int j = 0; j < 8; ++j) v[j] = (std::remove_reference_t<decltype(v[0])>)w0a(m, 16 * mt + r, dperm(ks, g, j));
+  } else {
+    const int e2 = e - kDOpA1;
+    const int l = e2 & 63, ks = (e2 >> 6) & 1, mt = e2 >> 7;
+    const int r = l & 15, g = l >> 4;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = (__bf16)w0a(m, dperm(ks, g, j), 16 * mt + r);
-      W.a4[mt][ks][l] = v;
-    }
+    for (int j = 0; j < 8; ++j) v[j] = (std::remove_reference_t<decltype(v[0])>)w0a(m, dperm(ks, g, j), 16 * mt + r);
   }
+  return v;
+}
+
+__device__ __forceinline__ void load_dict_weights(const ff_mlp_desc& m, DictW& W) {
+  for (int e = threadIdx.x; e < kDOpA1 + kDOpA4; e += blockDim.x) {
+    const dbf8 v = dict_operand<dbf8>(m, e);
+    if (e < kDOpA1) (&W.a1[0][0][0])[e] = v;
+    else (&W.a4[0][0][0])[e - kDOpA1] = v;
+  }
+}
+
+__device__ __forceinline__ void load_dict_w1f(const ff_mlp_desc& m, DictW1F& W) {
+  for (int e = threadIdx.x; e < kDOpA1; e += blockDim.x) (&W.a1[0][0][0])[e] = dict_operand<df8>(m, e);
+}
+
+// FP32 backward: the packed operand copy, written once per launch.
+__global__ __launch_bounds__(256) void dict_pack_f32_kernel(ff_mlp_desc m, df8* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < kDOpA1 + kDOpA4) out[e] = dict_operand<df8>(m, e);
 }
 
 // Gathered values of one pixel's pieces (lane-owned channels).
@@ -225,10 +320,11 @@ __device__ __forceinline__ void dict_gather(const ff_field_desc& fd, const float
 
 // Decoder forward for the wave's 16 pixels: returns the layer-1 accumulators
 // c1[hidden tile] and the output accumulator c2 (lane (p, g): outputs 4g..4g+3).
-__device__ __forceinline__ void dict_decoder_fwd(const DictW& W, const dbf8 (&a2)[2], const DictPix& P, int g,
+template <bool BF, class WR>
+__device__ __forceinline__ void dict_decoder_fwd(const WR& W, const DV<BF> (&a2)[2], const DictPix& P, int g,
                                                  df4 (&c1)[4], df4& c2) {
   const int lane = threadIdx.x & 63;
-  dbf8 xb[5];
+  DV<BF> xb[5];
 #pragma unroll
   for (int ks = 0; ks < 5; ++ks)
 #pragma unroll
@@ -237,34 +333,35 @@ __device__ __forceinline__ void dict_decoder_fwd(const DictW& W, const dbf8 (&a2
       float f;
       if (mt < kDMt) f = P.bas[mt][i] * P.cof[mt][i];
       else f = (g == 0 && i == 0) ? 1.f : 0.f;  // column 144: the bias
-      xb[ks][j] = (__bf16)f;
+      xb[ks][j] = dcvt<BF>(f);
     }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     df4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ks = 0; ks < 5; ++ks) acc = mfma16(W.a1[mt][ks][lane], xb[ks], acc);
+    for (int ks = 0; ks < 5; ++ks) acc = mfma16(W.a1(mt, ks, lane), xb[ks], acc);
     c1[mt] = acc;
   }
   df4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks) {
-    dbf8 hb;
+    DV<BF> hb;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) hb[j] = (__bf16)fmaxf(c1[2 * ks + (j >> 2)][j & 3], 0.f);
+    for (int j = 0; j < 8; ++j) hb[j] = dcvt<BF>(fmaxf(c1[2 * ks + (j >> 2)][j & 3], 0.f));
     acc = mfma16(a2[ks], hb, acc);
   }
   c2 = acc;
 }
 
-__device__ __forceinline__ void load_a2(const ff_mlp_desc& m, dbf8 (&a2)[2]) {
+template <bool BF>
+__device__ __forceinline__ void load_a2(const ff_mlp_desc& m, DV<BF> (&a2)[2]) {
   const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const float v = m.w1[(r < m.out_dim ? r : m.out_dim - 1) * kDH + dperm(ks, g, j)];
-      a2[ks][j] = (__bf16)(r < m.out_dim ? v : 0.f);
+      a2[ks][j] = dcvt<BF>(r < m.out_dim ? v : 0.f);
     }
 }
 
@@ -272,11 +369,15 @@ template <bool BF>
 __global__ __launch_bounds__(256) void dict_fwd_kernel(ff_field_desc fd, ff_mlp_desc m,
                                                        const float* __restrict__ x, int64_t n,
                                                        float* __restrict__ out) {
-  __shared__ DictW W;
-  load_dict_weights(m, W);
+  __shared__ std::conditional_t<BF, DictW, DictW1F> W;
+  if constexpr (BF)
+    load_dict_weights(m, W);
+  else
+    load_dict_w1f(m, W);
   __syncthreads();
-  dbf8 a2[2];
-  load_a2(m, a2);
+  const std::conditional_t<BF, DictWLds, DictW1Lds> WR{W};
+  DV<BF> a2[2];
+  load_a2<BF>(m, a2);
   const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const int O = m.out_dim;
   const int64_t ntiles = (n + kDPx - 1) / kDPx;
@@ -289,7 +390,7 @@ __global__ __launch_bounds__(256) void dict_fwd_kernel(ff_field_desc fd, ff_mlp_
     DictPix P;
     dict_gather<BF>(fd, xv, g, r, P, nullptr);
     df4 c1[4], c2;
-    dict_decoder_fwd(W, a2, P, g, c1, c2);
+    dict_decoder_fwd<BF>(WR, a2, P, g, c1, c2);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int o = 4 * g + i;
@@ -305,26 +406,34 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
                                                        const float* __restrict__ x, int64_t n,
                                                        const float* __restrict__ dsrc, int mode,
                                                        float loss_scale, float* __restrict__ ws,
-                                                       float* __restrict__ loss_sum) {
-  __shared__ DictW W;
-  __shared__ __attribute__((aligned(16))) DictWave WV[kDWaves];
+                                                       float* __restrict__ loss_sum, const df8* __restrict__ pack) {
+  // BF16: both operand matrices in LDS; FP32: the packed copy (the LDS area
+  // below is then the weights' size smaller)
+  __shared__ std::conditional_t<BF, DictW, char> W;
+  __shared__ __attribute__((aligned(16))) DictWave<BF> WV[kDWaves];
   __shared__ __attribute__((aligned(16))) DictCorners WK[kDWaves];
-  load_dict_weights(m, W);
-  __syncthreads();
+  if constexpr (BF) {
+    load_dict_weights(m, W);
+    __syncthreads();
+  }
+  const auto WR = [&] {
+    if constexpr (BF) return DictWLds{W};
+    else return DictWPack{pack};
+  }();
   const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4, l32 = lane & 31, h32 = lane >> 5;
   const int wib = threadIdx.x >> 6;
-  DictWave& S = WV[wib];
+  DictWave<BF>& S = WV[wib];
   DictCorners& K = WK[wib];
   const int O = m.out_dim;
-  dbf8 a2[2], a3[4];
-  load_a2(m, a2);
+  DV<BF> a2[2], a3[4];
+  load_a2<BF>(m, a2);
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int o = 8 * g + j;
       const float v = m.w1[(o < O ? o : O - 1) * kDH + 16 * mt + r];
-      a3[mt][j] = (__bf16)(o < O ? v : 0.f);
+      a3[mt][j] = dcvt<BF>(o < O ? v : 0.f);
     }
   df16 acc0[5][2], acc1[2];
 #pragma unroll
@@ -351,7 +460,7 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
     DictPix P;
     dict_gather<BF>(fd, xv, g, r, P, &K);
     df4 c1[4], c2;
-    dict_decoder_fwd(W, a2, P, g, c1, c2);
+    dict_decoder_fwd<BF>(WR, a2, P, g, c1, c2);
     // output gradient (lane (p, g) holds outputs 4g .. 4g+3)
     float d[4];
 #pragma unroll
@@ -375,9 +484,9 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
       d[i] = v;
     }
     // dH^T = W1^T . dout^T (outputs 0..3 sit in the g = 0 lanes), ReLU mask
-    dbf8 bd;
+    DV<BF> bd;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) bd[j] = (__bf16)((g == 0 && j < 4) ? d[j] : 0.f);
+    for (int j = 0; j < 8; ++j) bd[j] = dcvt<BF>((g == 0 && j < 4) ? d[j] : 0.f);
     df4 dh[4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
@@ -387,11 +496,11 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
       for (int i = 0; i < 4; ++i) t[i] = c1[mt][i] > 0.f ? t[i] : 0.f;
       dh[mt] = t;
     }
-    dbf8 bdh[2];
+    DV<BF> bdh[2];
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) bdh[ks][j] = (__bf16)dh[2 * ks + (j >> 2)][j & 3];
+      for (int j = 0; j < 8; ++j) bdh[ks][j] = dcvt<BF>(dh[2 * ks + (j >> 2)][j & 3]);
     // weight-gradient operands through LDS (phase A)
 #pragma unroll
     for (int mt = 0; mt < 10; ++mt)
@@ -400,32 +509,32 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
         float f;
         if (mt < kDMt) f = P.bas[mt][i] * P.cof[mt][i];
         else f = (g == 0 && i == 0) ? 1.f : 0.f;
-        S.a.xt[16 * mt + 4 * g + i][r] = (__bf16)(active ? f : 0.f);
+        S.a.xt[16 * mt + 4 * g + i][r] = dcvt<BF>(active ? f : 0.f);
       }
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        S.a.ht[16 * mt + 4 * g + i][r] = (__bf16)fmaxf(c1[mt][i], 0.f);
-        S.a.gt[16 * mt + 4 * g + i][r] = (__bf16)dh[mt][i];
+        S.a.ht[16 * mt + 4 * g + i][r] = dcvt<BF>(fmaxf(c1[mt][i], 0.f));
+        S.a.gt[16 * mt + 4 * g + i][r] = dcvt<BF>(dh[mt][i]);
       }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      S.a.dt[4 * g + i][r] = (__bf16)(g == 0 ? d[i] : 0.f);
-      S.a.dt[16 + 4 * g + i][r] = (__bf16)0.f;
+      S.a.dt[4 * g + i][r] = dcvt<BF>(g == 0 ? d[i] : 0.f);
+      S.a.dt[16 + 4 * g + i][r] = dcvt<BF>(0.f);
     }
     wave_lds_sync();
     {
-      const dbf8 bdt = *reinterpret_cast<const dbf8*>(&S.a.dt[l32][8 * h32]);
+      const DV<BF> bdt = lds_v8(&S.a.dt[l32][8 * h32]);
 #pragma unroll
       for (int nh = 0; nh < 2; ++nh) {
-        const dbf8 bg = *reinterpret_cast<const dbf8*>(&S.a.gt[32 * nh + l32][8 * h32]);
+        const DV<BF> bg = lds_v8(&S.a.gt[32 * nh + l32][8 * h32]);
 #pragma unroll
         for (int mk = 0; mk < 5; ++mk) {
-          const dbf8 ax = *reinterpret_cast<const dbf8*>(&S.a.xt[32 * mk + l32][8 * h32]);
+          const DV<BF> ax = lds_v8(&S.a.xt[32 * mk + l32][8 * h32]);
           acc0[mk][nh] = mfma32(ax, bg, acc0[mk][nh]);
         }
-        const dbf8 ah = *reinterpret_cast<const dbf8*>(&S.a.ht[32 * nh + l32][8 * h32]);
+        const DV<BF> ah = lds_v8(&S.a.ht[32 * nh + l32][8 * h32]);
         acc1[nh] = mfma32(ah, bdt, acc1[nh]);
       }
     }
@@ -435,7 +544,7 @@ __global__ __launch_bounds__(256) void dict_bwd_kernel(ff_field_desc fd, ff_mlp_
     for (int mt = 0; mt < kDMt; ++mt) {
       df4 t = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) t = mfma16(W.a4[mt][ks][lane], bdh[ks], t);
+      for (int ks = 0; ks < 2; ++ks) t = mfma16(WR.a4(mt, ks, lane), bdh[ks], t);
 #pragma unroll
       for (int i = 0; i < 4; ++i) df[mt][i] = t[i];
     }
@@ -614,7 +723,8 @@ int dict_partial(int O) { return kDH * kDC + kDH + O * kDH; }
 
 bool dict_supported(const ff_field_desc& d, const ff_mlp_desc& m) {
   static const int dims[6] = {32, 32, 32, 16, 16, 16};
-  if (d.table_dtype != FF_BF16) return false;  // the decoder runs in BF16: only for BF16 tables
+  // BF16 tables: BF16 decoder operands; FP32 tables: FP32 decoder operands
+  if (d.table_dtype != FF_BF16 && d.table_dtype != FF_F32) return false;
   if (d.in_dim != 2 || d.x_dim != 3 || d.n_levels != 6 || d.n_channels != kDC) return false;
   if (d.basis_kind != FF_BASIS_GRID || d.basis_interp != FF_INTERP_BILINEAR) return false;
   if (d.coeff_kind != FF_COEFF_GRID || d.coeff_ndim != 3 || d.coeff_interp != FF_INTERP_BILINEAR) return false;
@@ -625,24 +735,41 @@ bool dict_supported(const ff_field_desc& d, const ff_mlp_desc& m) {
          m.w0 && m.b0 && m.w1;
 }
 
-size_t dict_workspace_bytes(const ff_mlp_desc& m, int64_t n) {
-  if (n <= 0) return 0;
+// Workspace: the partial rows, then (FP32 form) the packed decoder operands,
+// 256-byte aligned.
+size_t dict_rows_bytes(const ff_mlp_desc& m, int64_t n) {
   return ((size_t)dict_blocks(n) * kDWaves + kReduceSplit) * dict_partial(m.out_dim) * sizeof(float);
+}
+
+size_t dict_workspace_bytes(const ff_field_desc& d, const ff_mlp_desc& m, int64_t n) {
+  if (n <= 0) return 0;
+  const size_t rows = dict_rows_bytes(m, n);
+  return d.table_dtype == FF_BF16 ? rows : (rows + 255) / 256 * 256 + kDPackBytes;
 }
 
 int launch_dict_fwd(const ff_field_desc& d, const ff_mlp_desc& m, const float* x, int64_t n, float* out,
                     hipStream_t s) {
-  hipLaunchKernelGGL(dict_fwd_kernel<true>, dim3(dict_blocks(n)), dim3(64 * kDWaves), 0, s, d, m, x, n, out);
+  if (d.table_dtype == FF_BF16)
+    hipLaunchKernelGGL(dict_fwd_kernel<true>, dim3(dict_blocks(n)), dim3(64 * kDWaves), 0, s, d, m, x, n, out);
+  else
+    hipLaunchKernelGGL(dict_fwd_kernel<false>, dim3(dict_blocks(n)), dim3(64 * kDWaves), 0, s, d, m, x, n, out);
   return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
 }
 
 int launch_dict_bwd(const ff_field_desc& d, const ff_mlp_desc& m, const ff_grads& g, const float* x, int64_t n,
                     const float* dsrc, int mode, float loss_scale, void* ws, size_t ws_bytes, float* loss_sum,
                     hipStream_t s) {
-  if (!ws || ws_bytes < dict_workspace_bytes(m, n)) return FF_ERR_WORKSPACE;
+  if (!ws || ws_bytes < dict_workspace_bytes(d, m, n)) return FF_ERR_WORKSPACE;
   const unsigned blocks = dict_blocks(n);
-  hipLaunchKernelGGL(dict_bwd_kernel<true>, dim3(blocks), dim3(64 * kDWaves), 0, s, d, m, g, x, n, dsrc, mode,
-                     loss_scale, (float*)ws, loss_sum);
+  if (d.table_dtype == FF_BF16) {
+    hipLaunchKernelGGL(dict_bwd_kernel<true>, dim3(blocks), dim3(64 * kDWaves), 0, s, d, m, g, x, n, dsrc, mode,
+                       loss_scale, (float*)ws, loss_sum, (const df8*)nullptr);
+  } else {
+    df8* pack = reinterpret_cast<df8*>(static_cast<char*>(ws) + (dict_rows_bytes(m, n) + 255) / 256 * 256);
+    hipLaunchKernelGGL(dict_pack_f32_kernel, dim3((kDOpA1 + kDOpA4 + 255) / 256), dim3(256), 0, s, m, pack);
+    hipLaunchKernelGGL(dict_bwd_kernel<false>, dim3(blocks), dim3(64 * kDWaves), 0, s, d, m, g, x, n, dsrc, mode,
+                       loss_scale, (float*)ws, loss_sum, (const df8*)pack);
+  }
   if (hipGetLastError() != hipSuccess) return FF_ERR_LAUNCH;
   return launch_reduce_partials((const float*)ws, (int)(blocks * kDWaves), 2, kDC, kDH, m.out_dim, g, s);
 }

@@ -379,7 +379,7 @@ int ff_field_fused_supported(const ff_field_desc* desc, const ff_mlp_desc* mlp) 
 
 size_t ff_field_bwd_workspace_bytes(const ff_field_desc* desc, const ff_mlp_desc* mlp, int64_t n) {
   if (check_desc(desc) || check_mlp(mlp) || n <= 0) return 0;
-  if (dict_supported(*desc, *mlp)) return dict_workspace_bytes(*mlp, n);
+  if (dict_supported(*desc, *mlp)) return dict_workspace_bytes(*desc, *mlp, n);
   return ff_linear_mat_bwd_workspace_bytes(mlp, n);
 }
 

@@ -38,7 +38,7 @@ int launch_mlp_bwd(const ff_mlp_desc& m, const float* f, int64_t n, const float*
 // Dictionary image-set fast path (ff_dict.hip): image-set mode, basis_dims
 // [32,32,32,16,16,16], BF16 tables, decoder 144 -> 64 -> O (O <= 4).
 bool dict_supported(const ff_field_desc& d, const ff_mlp_desc& m);
-size_t dict_workspace_bytes(const ff_mlp_desc& m, int64_t n);
+size_t dict_workspace_bytes(const ff_field_desc& d, const ff_mlp_desc& m, int64_t n);
 int launch_dict_fwd(const ff_field_desc& d, const ff_mlp_desc& m, const float* x, int64_t n, float* out,
                     hipStream_t s);
 int launch_dict_bwd(const ff_field_desc& d, const ff_mlp_desc& m, const ff_grads& g, const float* x, int64_t n,

@@ -1,7 +1,8 @@
 """Dictionary image-set fast path (image-set mode, basis_dims
-[32,32,32,16,16,16] = 144 channels, per-image coefficient grid, BF16 tables,
-decoder 144 -> 64 -> 3 on the matrix cores) against a CPU emulation of its
-arithmetic: features from the oracle on the BF16-rounded tables (FP32), the
+[32,32,32,16,16,16] = 144 channels, per-image coefficient grid, decoder
+144 -> 64 -> 3 on the matrix cores).  The FP32 form (FP32 tables and decoder
+operands) is checked against the FP32 oracle at 1e-5 / 1e-4 (end of file);
+the BF16 form against a CPU emulation of its arithmetic: features from the oracle on the BF16-rounded tables (FP32), the
 decoder's inputs (features with a constant-1 bias column), weights, hidden
 activations, output gradients and hidden gradients rounded to BF16 (round to
 nearest even), products accumulated in FP32.
@@ -144,3 +145,46 @@ def test_dict_batch_order_invariance():
         grads.append([p.grad.detach().clone() for p in params])
     for k, (a, b) in enumerate(zip(*grads)):
         assert_close(a, b, 1e-4, f"param {k}", atol_frac=1e-5)
+
+
+# ---------------------------------------------------------------- FP32 form
+def make_fp32():
+    print("[case] dict model, FP32", file=sys.stderr, flush=True)
+    m = new_model(small_cfg(**DICT), dtype="fp32", seed=2)
+    assert m.fused_supported()       # the dictionary kernels with FP32 tables and FP32 decoder operands
+    return m
+
+
+def test_dict_fp32_forward_matches_oracle():
+    """FP32 tables and the FP32 matrix-core decoder against the FP32 oracle
+    (models/FactorFields.py:425-533, :144-159) at 1e-5."""
+    m = make_fp32()
+    x = image_points(3001, seed=13)
+    out = m.field(x.cuda())
+    ref = oracle_of(m).field(x)
+    assert_close(out, ref, 1e-5, "FP32 dictionary forward")
+
+
+@pytest.mark.parametrize("how", ["regression_mse", "autograd", "regression_sorted"])
+def test_dict_fp32_backward_matches_oracle(how):
+    """Gradients of every table and decoder weight against the FP32 oracle's
+    autograd at 1e-4 (FP64 finite differences pin the oracle)."""
+    m = make_fp32()
+    n = 2503
+    x = cell_sorted_points(n, seed=15) if how == "regression_sorted" else image_points(n, seed=15)
+    tgt = torch.rand((n, 3), generator=torch.Generator().manual_seed(16))
+    r = torch.randn((n, 3), generator=torch.Generator().manual_seed(17))
+    m.zero_grad(set_to_none=True)
+    of = oracle_of(m).requires_grad_(True)
+    ref = of.field(x)
+    if how.startswith("regression"):
+        m.regression_step(x.cuda(), tgt.cuda(), loss="mse")
+        torch.mean((ref - tgt) ** 2).backward()
+    else:
+        (m.field(x.cuda()) * r.cuda()).sum().backward()
+        (ref * r).sum().backward()
+    ours = list(m.basises) + list(m.coeffs) + list(m.linear_mat.parameters())
+    theirs = of.basises + of.coeffs + [p for l in of.mlp for p in l if p is not None]
+    assert len(ours) == len(theirs)
+    for k, (t, o) in enumerate(zip(ours, theirs)):
+        assert_close(t.grad.contiguous(), o.grad, 1e-4, f"param {k}", atol_frac=1e-5)
