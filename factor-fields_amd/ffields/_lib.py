@@ -166,6 +166,9 @@ def _declare(lib):
         "ff_atomic_rate_probe": (ctypes.c_int, [c_vp, I64, I64, c_i32, c_vp]),
         "ff_camera_rays": (ctypes.c_int, [P(Cameras), c_vp, c_vp, I64, c_vp, c_vp]),
         "ff_gather_rays": (ctypes.c_int, [c_vp, c_vp, I64, c_vp, I64, c_vp, c_vp, c_vp]),
+        "ff_mt_count": (ctypes.c_int, [c_vp, I64, I64, I64, ctypes.c_double, c_i32, c_vp, c_vp]),
+        "ff_mt_emit": (ctypes.c_int, [c_vp, I64, I64, I64, ctypes.c_double, c_i32, c_vp, c_vp, c_vp]),
+        "ff_mt_verts": (ctypes.c_int, [c_vp, I64, I64, I64, ctypes.c_double, c_vp, I64, c_vp, c_vp]),
         "ff_composite_fwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp]),
         "ff_composite_bwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,

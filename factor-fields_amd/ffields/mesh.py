@@ -3,125 +3,59 @@ utils.py:178-249).
 
 The reference runs skimage's marching cubes on the dense alpha volume and
 writes the mesh with plyfile; neither library is available here.  This module
-extracts the same isosurface with marching tetrahedra: every grid cube is
-split into six tetrahedra along its main diagonal (the split is the same in
-neighbouring cubes, so the mesh is closed), each tetrahedron whose corners
-straddle the level contributes one or two triangles, and every vertex is the
-linear interpolation of the level on a grid edge, shared by all the triangles
-that use that edge.  The triangulation differs from marching cubes'; the
-surface, the vertex positions on the grid edges and the file layout are the
-reference's.  It runs with torch on the volume's device."""
-import itertools
-import struct
+extracts the same isosurface with marching tetrahedra on the GPU
+(ff_mt_count / ff_mt_emit / ff_mt_verts): every grid cube is split into six
+tetrahedra along its main diagonal (the split is the same in neighbouring
+cubes, so the mesh is closed), each tetrahedron whose corners straddle the
+level contributes one or two triangles, and every vertex is the linear
+interpolation of the level on a grid edge, shared by all the triangles that
+use that edge.  The triangulation differs from marching cubes'; the surface,
+the vertex positions on the grid edges and the file layout are the
+reference's."""
+import ctypes
 
 import torch
 
-# corners of a unit cube, corner c at offset (c & 1, (c >> 1) & 1, c >> 2) on
-# the volume's axes (0, 1, 2)
-_OFFS = [((c >> 0) & 1, (c >> 1) & 1, (c >> 2) & 1) for c in range(8)]
-
-
-def _kuhn_tets():
-    """The six tetrahedra of a cube that share its main diagonal: for every
-    order (p, q, r) of the axes, corners 0, e_p, e_p + e_q, 7."""
-    tets = []
-    for p, q, r in itertools.permutations(range(3)):
-        a = [0, 0, 0]
-        path = [tuple(a)]
-        for ax in (p, q):
-            a[ax] = 1
-            path.append(tuple(a))
-        path.append((1, 1, 1))
-        tets.append([_OFFS.index(v) for v in path])
-    return tets
-
-
-def _case_table():
-    """For each inside mask of a tetrahedron's 4 corners: up to two triangles,
-    each as 3 edges (u, v) between an inside corner u and an outside corner v."""
-    table = []
-    for m in range(16):
-        ins = [i for i in range(4) if (m >> i) & 1]
-        out = [i for i in range(4) if not (m >> i) & 1]
-        tris = []
-        if len(ins) == 1:
-            tris.append([(ins[0], v) for v in out])
-        elif len(ins) == 3:
-            tris.append([(u, out[0]) for u in ins])
-        elif len(ins) == 2:
-            a, b = ins
-            c, d = out
-            tris.append([(a, c), (a, d), (b, d)])
-            tris.append([(a, c), (b, d), (b, c)])
-        table.append(tris)
-    return table
-
-
-_TETS = _kuhn_tets()
-_CASES = _case_table()
+from . import _lib as L
 
 
 def marching_tetrahedra(vol, level, inside_above=True):
-    """Isosurface of a [A, B, C] volume at `level`.  Corners with value above
-    the level are inside when inside_above (density), below it otherwise
-    (signed distance).  Returns (verts [V, 3] float64 in index coordinates of
-    the volume's axes, faces [F, 3] int64), every face oriented with its
-    normal pointing from inside to outside."""
-    v = vol.to(torch.float64)
+    """Isosurface of a [A, B, C] volume (on the GPU; taken as FP32) at
+    `level`.  Corners with value above the level are inside when
+    inside_above (density), below it otherwise (signed distance).  Returns
+    (verts [V, 3] float64 in index coordinates of the volume's axes, sorted
+    by the grid edge they lie on, faces [F, 3] int64 in cube order), every
+    face oriented with its normal pointing from inside to outside.
+
+    Three launches and two torch steps between them: the triangle count of
+    every cube, its exclusive prefix sum (torch.cumsum), the edge keys of every
+    triangle written at its cube's offset, the distinct keys and every
+    corner's rank among them (torch.unique), and one vertex per distinct key."""
+    if not vol.is_cuda:
+        raise ValueError("marching_tetrahedra runs on the GPU: pass the volume as a CUDA tensor")
+    if vol.dim() != 3:
+        raise ValueError("expected a [A, B, C] volume")
+    v = vol.to(torch.float32).contiguous()
     dev = v.device
     A, B, C = v.shape
+    empty = (torch.zeros((0, 3), dtype=torch.float64, device=dev), torch.zeros((0, 3), dtype=torch.int64, device=dev))
     if min(A, B, C) < 2:
-        return torch.zeros((0, 3), dtype=torch.float64, device=dev), torch.zeros((0, 3), dtype=torch.int64, device=dev)
-    corner = torch.stack([v[o[0]:A - 1 + o[0], o[1]:B - 1 + o[1], o[2]:C - 1 + o[2]] for o in _OFFS])  # [8, ...]
-    ins = corner > level if inside_above else corner < level
-    active = ins.any(0) & (~ins).any(0)
-    cube = active.nonzero()                                                   # [M, 3] cube origins
-    if cube.shape[0] == 0:
-        return torch.zeros((0, 3), dtype=torch.float64, device=dev), torch.zeros((0, 3), dtype=torch.int64, device=dev)
-    offs = torch.tensor(_OFFS, device=dev)                                    # [8, 3]
-    tets = torch.tensor(_TETS, device=dev)                                    # [6, 4]
-    pos = cube[:, None, None, :] + offs[tets][None]                           # [M, 6, 4, 3] grid points
-    val = v[pos[..., 0], pos[..., 1], pos[..., 2]]                            # [M, 6, 4]
-    tin = val > level if inside_above else val < level
-    code = (tin.long() << torch.arange(4, device=dev)).sum(-1)                # [M, 6]
-    pos, val, tin, code = pos.reshape(-1, 4, 3), val.reshape(-1, 4), tin.reshape(-1, 4), code.reshape(-1)
-    pid = (pos[..., 0] * B + pos[..., 1]) * C + pos[..., 2]                   # [T, 4] grid point ids
-    faces, keys, ends = [], [], []
-    for m in range(1, 15):
-        sel = (code == m).nonzero().squeeze(1)
-        if sel.numel() == 0:
-            continue
-        for tri in _CASES[m]:
-            e = torch.tensor(tri, device=dev)                                 # [3, 2] (inside, outside) corners
-            pu, pv = pid[sel][:, e[:, 0]], pid[sel][:, e[:, 1]]               # [S, 3]
-            keys.append(torch.minimum(pu, pv) * (A * B * C) + torch.maximum(pu, pv))
-            ends.append(torch.stack([torch.minimum(pu, pv), torch.maximum(pu, pv)], -1))
-            # orientation: from the tetrahedron's inside corners to its outside ones
-            w = tin[sel].double()
-            cin = (pos[sel].double() * w[..., None]).sum(1) / w.sum(1, keepdim=True)
-            cout = (pos[sel].double() * (1 - w)[..., None]).sum(1) / (1 - w).sum(1, keepdim=True)
-            faces.append((sel, e, cout - cin))
-    key = torch.cat(keys).reshape(-1)
-    uniq, inv = torch.unique(key, return_inverse=True)
-    end = torch.cat(ends).reshape(-1, 2)
-    first = torch.full((uniq.numel(),), -1, dtype=torch.int64, device=dev)
-    first.scatter_(0, inv, torch.arange(inv.numel(), device=dev))
-    e0, e1 = end[first, 0], end[first, 1]                                     # endpoints, lower id first
-    flat = v.reshape(-1)
-    f0, f1 = flat[e0], flat[e1]
-
-    def coords(p):
-        return torch.stack([p // (B * C), (p // C) % B, p % C], -1).double()
-
-    t = ((level - f0) / (f1 - f0))[:, None]
-    verts = coords(e0) + t * (coords(e1) - coords(e0))
-    tri = inv.reshape(-1, 3)
-    # orient every face along its tetrahedron's inside -> outside direction
-    dirs = torch.cat([d for _, _, d in faces])
-    p0, p1, p2 = verts[tri[:, 0]], verts[tri[:, 1]], verts[tri[:, 2]]
-    flip = (torch.cross(p1 - p0, p2 - p0, dim=1) * dirs).sum(1) < 0
-    tri = torch.where(flip[:, None], tri[:, [0, 2, 1]], tri)
-    return verts, tri
+        return empty
+    lib, st, lev = L.lib(), L.stream_of(v), ctypes.c_double(float(level))
+    counts = torch.empty((A - 1) * (B - 1) * (C - 1), dtype=torch.int32, device=dev)
+    L.check(lib.ff_mt_count(L.ptr(v), A, B, C, lev, int(bool(inside_above)), L.ptr(counts), st), "ff_mt_count")
+    ends = torch.cumsum(counts, 0, dtype=torch.int64)
+    n_tri = int(ends[-1])
+    if n_tri == 0:
+        return empty
+    offsets = ends - counts
+    keys = torch.empty(3 * n_tri, dtype=torch.int64, device=dev)
+    L.check(lib.ff_mt_emit(L.ptr(v), A, B, C, lev, int(bool(inside_above)), L.ptr(offsets), L.ptr(keys), st),
+            "ff_mt_emit")
+    uniq, inv = torch.unique(keys, return_inverse=True)
+    verts = torch.empty((uniq.numel(), 3), dtype=torch.float64, device=dev)
+    L.check(lib.ff_mt_verts(L.ptr(v), A, B, C, lev, L.ptr(uniq), uniq.numel(), L.ptr(verts), st), "ff_mt_verts")
+    return verts, inv.reshape(-1, 3)
 
 
 def write_ply(path, verts, faces):

@@ -523,6 +523,33 @@ int ff_camera_rays(const ff_cameras* cams, const int64_t* view, const int64_t* p
 int ff_gather_rays(const float* rays, const float* rgbs, int64_t n_total, const int64_t* idx, int64_t B,
                    float* out_rays, float* out_rgbs, void* stream);
 
+/* Isosurface extraction of a dense [A, B, C] FP32 volume (row-major, point
+ * (i, j, k) at vol[(i * B + j) * C + k]) at `level`, in three launches on
+ * `stream`.  Replaces skimage.measure.marching_cubes in
+ * convert_sdf_samples_to_ply (utils.py:203-205; called from
+ * train_per_scene.py:32-41) by marching tetrahedra: each cube splits into
+ * the six tetrahedra around its (0,0,0)-(1,1,1) diagonal, each tetrahedron
+ * straddling the level gives one or two triangles.  Inside is value > level
+ * when inside_above != 0 (densities), value < level otherwise (distances).
+ * A, B, C >= 2, each <= 2^20, (A B C)^2 < 9e18.
+ *
+ * ff_mt_count: counts[c] = triangles of cube c, c = (i * (B-1) + j) * (C-1)
+ *   + k, for the (A-1)(B-1)(C-1) cubes.
+ * ff_mt_emit: offsets[c] = exclusive prefix sum of counts; writes three edge
+ *   keys per triangle at keys[3 * offsets[c] ...]: key = lo * (A B C) + hi
+ *   for the edge between grid points lo < hi, ordered so that the normal
+ *   points from the tetrahedron's inside corners to its outside ones.
+ * ff_mt_verts: for n distinct keys, verts[i] = the FP64 point in index
+ *   coordinates where the level crosses edge keys[i] (linear in the two
+ *   endpoint values, from the lower point).  The caller deduplicates the
+ *   emitted keys (a sort) and maps every triangle corner to its key's rank. */
+int ff_mt_count(const float* vol, int64_t A, int64_t B, int64_t C, double level, int32_t inside_above,
+                int32_t* counts, void* stream);
+int ff_mt_emit(const float* vol, int64_t A, int64_t B, int64_t C, double level, int32_t inside_above,
+               const int64_t* offsets, int64_t* keys, void* stream);
+int ff_mt_verts(const float* vol, int64_t A, int64_t B, int64_t C, double level, const int64_t* keys, int64_t n,
+                double* verts, void* stream);
+
 /* Measurement only (no counterpart in the reference): the FP32 atomic
  * line-request rate that bounds every backward's table scatter.  Issues
  * n_instr wave instructions of global_atomic_add_f32 (+1.0), each touching

@@ -45,6 +45,10 @@ How each function is pinned (checked against something other than itself):
                            axis permutations under 90-degree rotations, unit
                            norms, origins equal to the translation
                            (tests/test_oracle_rays.py).
+* marching tetrahedra   -- a sphere's signed distance: vertices on the sphere,
+  (oracle/mesh.py)         a closed, consistently oriented mesh, area and
+                           enclosed volume against the closed forms
+                           (tests/test_mesh.py).
 
 tests/golden/ holds fixtures written from these functions
 (tests/golden/make_golden.py); tests/test_golden.py recomputes them.

@@ -185,3 +185,23 @@ def test_ray_entry_points_validate_arguments():
     assert lib.ff_gather_rays(P(0x1000), None, 10, None, 4, P(0x3000), None, None) == L.FF_ERR_ARG      # no idx
     assert lib.ff_gather_rays(P(0x1000), None, -1, P(0x2000), 4, P(0x3000), None, None) == L.FF_ERR_ARG
     assert lib.ff_gather_rays(P(0x1000), None, 10, P(0x2000), -4, P(0x3000), None, None) == L.FF_ERR_ARG
+
+
+def test_mesh_entry_points_validate_arguments():
+    """ff_mt_count / ff_mt_emit / ff_mt_verts: volumes under 2 points on an
+    axis, over 2^20, or whose edge keys would overflow int64, and missing
+    buffers are rejected before any launch; no distinct keys is a no-op."""
+    lib = L.lib()
+    P = ctypes.c_void_p
+    lev = ctypes.c_double(0.5)
+    assert lib.ff_mt_count(None, 4, 4, 4, lev, 1, P(0x2000), None) == L.FF_ERR_ARG
+    assert lib.ff_mt_count(P(0x1000), 1, 4, 4, lev, 1, P(0x2000), None) == L.FF_ERR_ARG
+    assert lib.ff_mt_count(P(0x1000), 4, 4, (1 << 20) + 1, lev, 1, P(0x2000), None) == L.FF_ERR_ARG
+    assert lib.ff_mt_count(P(0x1000), 1 << 20, 1 << 20, 4, lev, 1, P(0x2000), None) == L.FF_ERR_ARG  # keys overflow
+    assert lib.ff_mt_count(P(0x1000), 4, 4, 4, lev, 1, None, None) == L.FF_ERR_ARG
+    assert lib.ff_mt_emit(P(0x1000), 4, 4, 4, lev, 1, None, P(0x3000), None) == L.FF_ERR_ARG
+    assert lib.ff_mt_emit(P(0x1000), 4, 4, 4, lev, 1, P(0x2000), None, None) == L.FF_ERR_ARG
+    assert lib.ff_mt_verts(P(0x1000), 4, 4, 4, lev, None, 0, None, None) == L.FF_OK
+    assert lib.ff_mt_verts(P(0x1000), 4, 4, 4, lev, P(0x2000), -1, P(0x3000), None) == L.FF_ERR_ARG
+    assert lib.ff_mt_verts(P(0x1000), 4, 4, 4, lev, None, 5, P(0x3000), None) == L.FF_ERR_ARG
+    assert lib.ff_mt_verts(P(0x1000), 0, 4, 4, lev, P(0x2000), 5, P(0x3000), None) == L.FF_ERR_ARG

@@ -1,12 +1,15 @@
 """Isosurface extraction and PLY export (train_per_scene.py:32-41,
-utils.py:178-249): a sphere's signed distance gives a closed, consistently
-oriented mesh whose vertices lie on the sphere and whose area and enclosed
-volume match the closed forms; the PLY file round-trips."""
+utils.py:178-249), on the CPU: the oracle's marching tetrahedra on a sphere's
+signed distance give a closed, consistently oriented mesh whose vertices lie
+on the sphere and whose area and enclosed volume match the closed forms; the
+PLY file round-trips.  The device path is compared with this oracle in
+tests/test_gpu_mesh.py."""
 import math
 
 import torch
 
-from ffields.mesh import marching_tetrahedra, read_ply, write_ply
+from ffields.mesh import read_ply, write_ply
+from oracle.mesh import marching_tetrahedra
 
 
 def sphere(n=40, r=0.6):

@@ -62,7 +62,12 @@ def test_marchcude_to_world():
     assert torch.allclose(w, torch.tensor([[-1.0, -1, -1], [1, 1, 1], [0, 0, 0]], dtype=torch.float64))
 
 
-def test_export_sdf_mesh(tmp_path):
+def test_export_sdf_mesh(tmp_path, monkeypatch):
+    """The export's wiring (grid, world coordinates, PLY) on the CPU, with the
+    oracle's marching tetrahedra in place of the device kernels."""
+    import ffields.sdf
+    import oracle.mesh
+    monkeypatch.setattr(ffields.sdf, "marching_tetrahedra", oracle.mesh.marching_tetrahedra)
     R = 0.6
     m = Analytic(lambda x: (x - 1.0).norm(dim=1) - R)                # sphere at the centre of [0, 2]^3
     reso = (33, 33, 33)
