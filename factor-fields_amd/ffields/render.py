@@ -588,13 +588,13 @@ class RenderMixin:
         occ = torch.empty((Z, Y, X), dtype=torch.uint8, device=self.device)
         L.check(L.lib().ff_occupancy_pool(L.ptr(alpha), Z, Y, X, 1.0, thres, L.ptr(occ),
                                           L.stream_of(alpha)), "ff_occupancy_pool")
-        keep = _remove_small_objects(occ.cpu().numpy().astype(bool), min_size=float(np.mean([Z, Y, X])))
-        if not keep.any():
+        keep = remove_small_components(occ, min_size=float(np.mean([Z, Y, X])))
+        if not bool(keep.any()):
             # Nothing above the threshold yet (an untrained field): the reference
             # would fail on the empty reduction below; keep the current mask and box.
             warnings.warn("updateAlphaMask: no voxel above the threshold; mask and box unchanged")
             return self.aabb.clone()
-        alpha = torch.from_numpy(keep.astype(np.float32)).to(self.device)
+        alpha = keep.float()
         if is_update_alphaMask:
             self.alphaMask = AlphaGridMask(self.device, self.inward_aabb, alpha)
         valid_xyz = dense_xyz[alpha > 0.5]
@@ -655,19 +655,20 @@ def _buf(buf, key, shape, device):
     return t
 
 
-def _remove_small_objects(mask, min_size):
+def remove_small_components(mask, min_size):
     """skimage.morphology.remove_small_objects(mask, min_size, connectivity=1)
-    (models/FactorFields.py:774-775): drop face-connected components with fewer
-    than min_size voxels."""
-    from scipy import ndimage
-    lab, n = ndimage.label(mask, structure=ndimage.generate_binary_structure(mask.ndim, 1))
-    if n == 0:
-        return mask
-    sizes = np.bincount(lab.ravel())
-    small = sizes < min_size
-    small[0] = False
-    out = mask.copy()
-    out[small[lab]] = False
+    (models/FactorFields.py:774-775) on the GPU by ff_remove_small_components:
+    drop face-connected components with fewer than min_size voxels.  mask is
+    a [D, H, W] CUDA tensor (non-zero = set); returns a uint8 tensor of 0/1."""
+    if not mask.is_cuda or mask.dim() != 3:
+        raise ValueError("remove_small_components takes a [D, H, W] CUDA tensor")
+    m = (mask != 0).to(torch.uint8).contiguous()
+    D, H, W = m.shape
+    labels = torch.empty(m.shape, dtype=torch.int32, device=m.device)
+    sizes = torch.empty(m.shape, dtype=torch.int32, device=m.device)
+    out = torch.empty_like(m)
+    L.check(L.lib().ff_remove_small_components(L.ptr(m), D, H, W, float(min_size), L.ptr(labels), L.ptr(sizes),
+                                               L.ptr(out), L.stream_of(m)), "ff_remove_small_components")
     return out
 
 

@@ -523,6 +523,18 @@ int ff_camera_rays(const ff_cameras* cams, const int64_t* view, const int64_t* p
 int ff_gather_rays(const float* rays, const float* rgbs, int64_t n_total, const int64_t* idx, int64_t B,
                    float* out_rays, float* out_rgbs, void* stream);
 
+/* The small-object filter of updateAlphaMask: skimage.morphology.
+ * remove_small_objects(mask, min_size, connectivity=1) at
+ * models/FactorFields.py:774-775.  mask [D, H, W] uint8 (non-zero = set);
+ * out[i] = 1 where voxel i is set and its face-connected component has at
+ * least min_size voxels, 0 elsewhere.  labels and sizes are caller-given
+ * int32 work buffers of D * H * W entries; on return labels[i] is the
+ * smallest index in voxel i's component (-1 for unset voxels) and
+ * sizes[labels[i]] that component's voxel count.  D H W <= 2^31 - 1.
+ * Five launches on `stream`. */
+int ff_remove_small_components(const uint8_t* mask, int32_t D, int32_t H, int32_t W, double min_size,
+                               int32_t* labels, int32_t* sizes, uint8_t* out, void* stream);
+
 /* Isosurface extraction of a dense [A, B, C] FP32 volume (row-major, point
  * (i, j, k) at vol[(i * B + j) * C + k]) at `level`, in three launches on
  * `stream`.  Replaces skimage.measure.marching_cubes in

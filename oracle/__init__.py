@@ -45,6 +45,9 @@ How each function is pinned (checked against something other than itself):
                            axis permutations under 90-degree rotations, unit
                            norms, origins equal to the translation
                            (tests/test_oracle_rays.py).
+* remove_small_objects  -- a brute-force 6-neighbour flood fill on random
+                           masks, diagonal contact left unconnected
+                           (tests/test_occupancy_cpu.py).
 * marching tetrahedra   -- a sphere's signed distance: vertices on the sphere,
   (oracle/mesh.py)         a closed, consistently oriented mesh, area and
                            enclosed volume against the closed forms

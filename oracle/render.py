@@ -199,3 +199,30 @@ def sample_pdf(bins, weights, N_samples, det=False, u=None):
     denom = torch.where(denom < 1e-5, torch.ones_like(denom), denom)
     t = (u - cdf_g[..., 0]) / denom
     return bins_g[..., 0] + t * (bins_g[..., 1] - bins_g[..., 0])
+
+
+def remove_small_objects(mask, min_size):
+    """skimage.morphology.remove_small_objects(mask, min_size, connectivity=1)
+    as updateAlphaMask calls it (models/FactorFields.py:774-775): label the
+    face-connected components (scipy.ndimage.label with the 6-neighbour
+    structure) and unset the voxels of components with fewer than min_size
+    voxels.  mask: numpy bool [D, H, W].  Also returns, for comparisons of
+    the labelling itself, each voxel's component as the smallest flat index
+    in it (-1 for unset voxels) and each voxel's component size (0 for unset
+    voxels)."""
+    import numpy as np
+    from scipy import ndimage
+    lab, n = ndimage.label(mask, structure=ndimage.generate_binary_structure(mask.ndim, 1))
+    flat = lab.ravel()
+    sizes = np.bincount(flat, minlength=n + 1)
+    first = np.full(n + 1, np.iinfo(np.int64).max, dtype=np.int64)
+    np.minimum.at(first, flat, np.arange(flat.size, dtype=np.int64))
+    first[0] = -1
+    small = sizes < min_size
+    small[0] = False
+    out = mask.copy()
+    out[small[lab]] = False
+    voxel_sizes = sizes[lab]
+    voxel_sizes[lab == 0] = 0
+    return out, first[lab], voxel_sizes
+

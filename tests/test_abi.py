@@ -205,3 +205,18 @@ def test_mesh_entry_points_validate_arguments():
     assert lib.ff_mt_verts(P(0x1000), 4, 4, 4, lev, P(0x2000), -1, P(0x3000), None) == L.FF_ERR_ARG
     assert lib.ff_mt_verts(P(0x1000), 4, 4, 4, lev, None, 5, P(0x3000), None) == L.FF_ERR_ARG
     assert lib.ff_mt_verts(P(0x1000), 0, 4, 4, lev, P(0x2000), 5, P(0x3000), None) == L.FF_ERR_ARG
+
+
+def test_component_filter_validates_arguments():
+    """ff_remove_small_components: an empty volume is a no-op with NULL
+    buffers; negative sizes, more than 2^31 - 1 voxels, missing buffers and a
+    NaN min_size are rejected before any launch."""
+    lib = L.lib()
+    P = ctypes.c_void_p
+    bufs = (P(0x2000), P(0x3000), P(0x4000))
+    assert lib.ff_remove_small_components(None, 0, 5, 5, 3.0, None, None, None, None) == L.FF_OK
+    assert lib.ff_remove_small_components(P(0x1000), -1, 5, 5, 3.0, *bufs, None) == L.FF_ERR_ARG
+    assert lib.ff_remove_small_components(P(0x1000), 2048, 1024, 1024, 3.0, *bufs, None) == L.FF_ERR_ARG
+    assert lib.ff_remove_small_components(None, 4, 5, 5, 3.0, *bufs, None) == L.FF_ERR_ARG
+    assert lib.ff_remove_small_components(P(0x1000), 4, 5, 5, 3.0, P(0x2000), None, P(0x4000), None) == L.FF_ERR_ARG
+    assert lib.ff_remove_small_components(P(0x1000), 4, 5, 5, float("nan"), *bufs, None) == L.FF_ERR_ARG

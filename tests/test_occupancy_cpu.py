@@ -1,9 +1,15 @@
 """CPU test of the small-object filter used by updateAlphaMask
 (models/FactorFields.py:774-775, skimage remove_small_objects with
-connectivity=1), checked against a brute-force 6-neighbour flood fill."""
+connectivity=1): the oracle's restatement checked against a brute-force
+6-neighbour flood fill.  The device filter is compared with the oracle in
+tests/test_gpu_occupancy.py."""
 import numpy as np
 
-from ffields.render import _remove_small_objects
+from oracle.render import remove_small_objects
+
+
+def _remove_small_objects(mask, min_size):
+    return remove_small_objects(mask, min_size)[0]
 
 
 def _flood(mask, min_size):
@@ -31,12 +37,33 @@ def _flood(mask, min_size):
     return out
 
 
+def _component(mask, seed):
+    seen, stack = {seed}, [seed]
+    while stack:
+        c = stack.pop()
+        for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            n = tuple(int(c[k] + d[k]) for k in range(3))
+            if all(0 <= n[k] < mask.shape[k] for k in range(3)) and mask[n] and n not in seen:
+                seen.add(n)
+                stack.append(n)
+    return seen
+
+
 def test_remove_small_objects_matches_flood_fill():
     rng = np.random.default_rng(0)
     for trial in range(5):
         m = rng.random((9, 11, 7)) < 0.35
         ms = float(np.mean(m.shape))
         assert np.array_equal(_remove_small_objects(m, ms), _flood(m, ms))
+        _, first, size = remove_small_objects(m, ms)
+        # the component labelling: smallest flat index of the component, its size
+        kept = _flood(m, 1)
+        assert np.array_equal(first >= 0, kept) and np.array_equal(size > 0, kept)
+        z, y, x = np.nonzero(m)
+        for a, b, c in zip(z[:20], y[:20], x[:20]):
+            comp = _component(m, (a, b, c))
+            assert first[a, b, c] == min(np.ravel_multi_index(v, m.shape) for v in comp)
+            assert size[a, b, c] == len(comp)
 
 
 def test_diagonal_contact_is_not_connected():
