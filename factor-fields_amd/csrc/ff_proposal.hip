@@ -333,6 +333,63 @@ __global__ __launch_bounds__(64) void proposal_bwd_kernel(ff_proposal_desc p, co
   if (lane == 0 && loss_sum) unsafeAtomicAdd(loss_sum, lsum * loss_scale / (float)B);
 }
 
+// Sample offsets of the proposal's rays in one launch: n_fine for a ray that
+// crosses the box, 0 otherwise (the same test as the sampling pass), then
+// the exclusive prefix sum over rays, in one block.  Replaces a counting
+// pass of one wave per ray and a separate scan for batches up to
+// kPropOffsetsMaxRays rays.
+constexpr int64_t kPropOffsetsMaxRays = 1 << 16;
+
+__global__ __launch_bounds__(1024) void proposal_offsets_kernel(ff_proposal_desc p, const float* __restrict__ rays,
+                                                                int64_t B, int32_t* __restrict__ off) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (B + 1023) / 1024;
+  const int64_t lo = t * per, hi = lo + per < B ? lo + per : B;
+  const int nf = p.n_fine;
+  int32_t s = 0;
+  for (int64_t b = lo; b < hi; ++b) {
+    PropRay r;
+    prop_ray(p, rays, b, r);
+    s += r.hit ? nf : 0;
+  }
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int32_t v = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int32_t run = t ? part[t - 1] : 0;
+  for (int64_t b = lo; b < hi; ++b) {
+    PropRay r;
+    prop_ray(p, rays, b, r);
+    run += r.hit ? nf : 0;
+    off[b + 1] = run;
+  }
+  if (t == 0) off[0] = 0;
+}
+
+// sum((a - b)^2) over n floats added to out[0], one block, fixed order.
+__global__ __launch_bounds__(256) void sq_err_sum_kernel(const float* __restrict__ a, const float* __restrict__ b,
+                                                         int64_t n, float* __restrict__ out) {
+  __shared__ float part[256 / kWave];
+  float s = 0.f;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const float d = a[i] - b[i];
+    s = fmaf(d, d, s);
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float tot = 0.f;
+    for (int w = 0; w < 256 / kWave; ++w) tot += part[w];
+    out[0] += tot;
+  }
+}
+
 int64_t prop_bwd_waves(int64_t n_rays) {
   int cu = 256, dev = 0;
   if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -368,10 +425,14 @@ int ff_proposal_sample(const ff_proposal_desc* p, const float* rays, const float
   const ff_proposal_desc pd = *p;
   auto go = [&](auto BFi) {
     constexpr bool BF = decltype(BFi)::value;
-    hipLaunchKernelGGL((proposal_sample_kernel<BF>), dim3(blocks), dim3(256), 0, s, pd, rays, prop_u, fine_u,
-                       n_rays, 0, *out);
-    int r = launch_scan_offsets(out->ray_off, n_rays, s);
-    if (r) return r;
+    if (n_rays <= kPropOffsetsMaxRays) {
+      hipLaunchKernelGGL(proposal_offsets_kernel, dim3(1), dim3(1024), 0, s, pd, rays, n_rays, out->ray_off);
+    } else {
+      hipLaunchKernelGGL((proposal_sample_kernel<BF>), dim3(blocks), dim3(256), 0, s, pd, rays, prop_u, fine_u,
+                         n_rays, 0, *out);
+      int r = launch_scan_offsets(out->ray_off, n_rays, s);
+      if (r) return r;
+    }
     hipLaunchKernelGGL((proposal_sample_kernel<BF>), dim3(blocks), dim3(256), 0, s, pd, rays, prop_u, fine_u,
                        n_rays, 1, *out);
     return hipGetLastError() == hipSuccess ? (int)FF_OK : (int)FF_ERR_LAUNCH;
@@ -412,6 +473,14 @@ int ff_proposal_bwd(const ff_proposal_desc* p, const float* rays, const float* p
   mg.b0 = grads->b0;
   mg.w1 = grads->w1;
   return launch_reduce_partials(wsf, (int)waves, 2, kPC, kPH, 1, mg, s);
+}
+
+int ff_sq_err_sum(const float* a, const float* b, int64_t n, float* out, void* stream) {
+  if (n < 0 || !out) return FF_ERR_ARG;
+  if (n == 0) return FF_OK;
+  if (!a || !b) return FF_ERR_ARG;
+  hipLaunchKernelGGL(sq_err_sum_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, a, b, n, out);
+  return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
 }
 
 }  // extern "C"

@@ -166,6 +166,7 @@ def _declare(lib):
         "ff_atomic_rate_probe": (ctypes.c_int, [c_vp, I64, I64, c_i32, c_vp]),
         "ff_camera_rays": (ctypes.c_int, [P(Cameras), c_vp, c_vp, I64, c_vp, c_vp]),
         "ff_gather_rays": (ctypes.c_int, [c_vp, c_vp, I64, c_vp, I64, c_vp, c_vp, c_vp]),
+        "ff_sq_err_sum": (ctypes.c_int, [c_vp, c_vp, I64, c_vp, c_vp]),
         "ff_remove_small_components": (ctypes.c_int, [c_vp, c_i32, c_i32, c_i32, ctypes.c_double, c_vp, c_vp,
                                                        c_vp, c_vp]),
         "ff_mt_count": (ctypes.c_int, [c_vp, I64, I64, I64, ctypes.c_double, c_i32, c_vp, c_vp]),

@@ -106,6 +106,15 @@ class ProposalNet(nn.Module):
                 "ff_proposal_bwd")
 
 
+def _sq_err_add(a, b, out):
+    """out += sum((a - b)^2) on the current stream (ff_sq_err_sum, one launch)."""
+    a = a.contiguous()
+    b = b.to(a.dtype).contiguous()
+    if a.shape != b.shape:
+        raise ValueError("shapes differ")
+    L.check(L.lib().ff_sq_err_sum(L.ptr(a), L.ptr(b), a.numel(), L.ptr(out), L.stream_of(a)), "ff_sq_err_sum")
+
+
 class ProposalReferee:
     """The occupancy sampler as fallback and referee of the learned proposal.
 
@@ -143,7 +152,7 @@ class ProposalReferee:
         with torch.cuda.stream(self.stream):
             self.stream.wait_event(ready)
             occ = model.render_eval(jr, use_proposal=False, N_samples=self.n_samples)
-            self.sq_err[0:1].add_(F.mse_loss(occ, jt, reduction="sum"))
+            _sq_err_add(occ, jt, self.sq_err[0:1])
             self._done = torch.cuda.Event()
             self._done.record(self.stream)
         # the batch tensors are read on the side stream: keep their memory until it is done
@@ -158,7 +167,7 @@ class ProposalReferee:
             return
         self._pending = False
         n = self.n_rays
-        self.sq_err[1:2].add_(F.mse_loss(rgb_map[:n], target[:n], reduction="sum"))
+        _sq_err_add(rgb_map[:n], target[:n], self.sq_err[1:2])
 
     def join(self):
         if self._done is not None:

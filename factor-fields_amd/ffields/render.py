@@ -465,10 +465,11 @@ class RenderMixin:
         target = target.contiguous()
         B = rays.shape[0]
         nf = pr.n_fine
-        if prop_u is None:
-            prop_u = torch.rand((B, 64), device=rays.device)
-        if fine_u is None:
-            fine_u = torch.rand((B, nf), device=rays.device)
+        if prop_u is None or fine_u is None:
+            # both draws from one launch: two contiguous slices of one buffer
+            u = torch.rand(B * (64 + nf), device=rays.device)
+            prop_u = u[:B * 64].view(B, 64) if prop_u is None else prop_u
+            fine_u = u[B * 64:].view(B, nf) if fine_u is None else fine_u
         bg = bool(white_bg or torch.rand((1,)) < 0.5)
         buf = self.render_buffers(B, nf)
         s = _samples_struct(buf)

@@ -523,6 +523,13 @@ int ff_camera_rays(const ff_cameras* cams, const int64_t* view, const int64_t* p
 int ff_gather_rays(const float* rays, const float* rgbs, int64_t n_total, const int64_t* idx, int64_t B,
                    float* out_rays, float* out_rgbs, void* stream);
 
+/* The proposal referee's squared error (no counterpart in the reference,
+ * which has no proposal network): out[0] += sum_i (a[i] - b[i])^2 over n
+ * floats, one block summing in a fixed order.  Used for the per-batch
+ * comparison of the occupancy sampler's and the proposal's renders of the
+ * same rays. */
+int ff_sq_err_sum(const float* a, const float* b, int64_t n, float* out, void* stream);
+
 /* The small-object filter of updateAlphaMask: skimage.morphology.
  * remove_small_objects(mask, min_size, connectivity=1) at
  * models/FactorFields.py:774-775.  mask [D, H, W] uint8 (non-zero = set);

@@ -220,3 +220,12 @@ def test_component_filter_validates_arguments():
     assert lib.ff_remove_small_components(None, 4, 5, 5, 3.0, *bufs, None) == L.FF_ERR_ARG
     assert lib.ff_remove_small_components(P(0x1000), 4, 5, 5, 3.0, P(0x2000), None, P(0x4000), None) == L.FF_ERR_ARG
     assert lib.ff_remove_small_components(P(0x1000), 4, 5, 5, float("nan"), *bufs, None) == L.FF_ERR_ARG
+
+
+def test_sq_err_sum_validates_arguments():
+    lib = L.lib()
+    P = ctypes.c_void_p
+    assert lib.ff_sq_err_sum(None, None, 0, P(0x3000), None) == L.FF_OK
+    assert lib.ff_sq_err_sum(P(0x1000), P(0x2000), 4, None, None) == L.FF_ERR_ARG
+    assert lib.ff_sq_err_sum(P(0x1000), None, 4, P(0x3000), None) == L.FF_ERR_ARG
+    assert lib.ff_sq_err_sum(P(0x1000), P(0x2000), -1, P(0x3000), None) == L.FF_ERR_ARG

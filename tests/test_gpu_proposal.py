@@ -7,6 +7,8 @@ sum on the GPU and a sequential cumsum on the CPU, and sample_pdf's
 `denom < 1e-5 -> 1` rule makes the inverse CDF discontinuous at near-empty
 bins, so a draw within rounding of a CDF step can land a bin away (at most 1%
 of the samples); gradients within 1e-4 relative."""
+import sys
+
 import pytest
 import torch
 
@@ -79,6 +81,26 @@ def test_proposal_sample_matches_oracle(train, dtype):
     xyz = buf["xyz"][:M].cpu().view(-1, 48, 3)
     r = rays[hit]
     assert_close(xyz, r[:, None, :3] + r[:, None, 3:6] * t_gpu[..., None], 1e-5, "xyz", atol_frac=1e-6)
+
+
+def test_proposal_offsets_large_batch():
+    """Above 65,536 rays the offsets come from the counting pass and the
+    separate scan; they equal the one-launch offsets (pinned by the oracle
+    test above) of the same rays taken in chunks."""
+    print("[case] proposal offsets, 70,000 rays", file=sys.stderr, flush=True)
+    p = make_prop(dtype="bf16")
+    B = 70_000
+    rays = make_rays(B, seed=33)
+    rays[::7, 3:6] = -rays[::7, 3:6]                  # misses scattered through the batch
+    big, _ = run_sample(p, rays, None, None)
+    counts = []
+    for lo in range(0, B, 40_000):
+        part, _ = run_sample(p, rays[lo:lo + 40_000], None, None)
+        off = part["ray_off"].cpu()
+        counts.append(off[1:] - off[:-1])
+    want = torch.cat([torch.zeros(1, dtype=torch.int32), torch.cumsum(torch.cat(counts), 0).int()])
+    assert torch.equal(big["ray_off"].cpu(), want)
+    assert int(want[-1]) > 0 and int(want[-1]) < 48 * B
 
 
 def test_proposal_backward_matches_oracle():
