@@ -342,31 +342,47 @@ constexpr int64_t kPropOffsetsMaxRays = 1 << 16;
 
 __global__ __launch_bounds__(1024) void proposal_offsets_kernel(ff_proposal_desc p, const float* __restrict__ rays,
                                                                 int64_t B, int32_t* __restrict__ off) {
-  __shared__ int32_t part[1024];
-  const int t = threadIdx.x;
+  static_assert(kPropOffsetsMaxRays <= 64 * 1024, "one 64-bit hit mask per thread");
+  __shared__ int32_t wsum[1024 / kWave];
+  const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
   const int64_t per = (B + 1023) / 1024;
-  const int64_t lo = t * per, hi = lo + per < B ? lo + per : B;
+  const int64_t lo = t * per;
   const int nf = p.n_fine;
+  uint64_t hits = 0;
   int32_t s = 0;
-  for (int64_t b = lo; b < hi; ++b) {
+  for (int64_t j = 0; j < per; ++j) {
+    if (lo + j >= B) break;
     PropRay r;
-    prop_ray(p, rays, b, r);
-    s += r.hit ? nf : 0;
+    prop_ray(p, rays, lo + j, r);
+    if (r.hit) {
+      hits |= 1ull << j;
+      s += nf;
+    }
   }
-  part[t] = s;
+  // block-wide inclusive scan: in each wave, then over the wave totals
+  int32_t incl = s;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int32_t v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  if (lane == kWave - 1) wsum[wv] = incl;
   __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const int32_t v = t >= d ? part[t - d] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  if (wv == 0) {
+    int32_t x = lane < 1024 / kWave ? wsum[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < 1024 / kWave; o <<= 1) {
+      const int32_t v = __shfl_up(x, o);
+      if (lane >= o) x += v;
+    }
+    if (lane < 1024 / kWave) wsum[lane] = x;
   }
-  int32_t run = t ? part[t - 1] : 0;
-  for (int64_t b = lo; b < hi; ++b) {
-    PropRay r;
-    prop_ray(p, rays, b, r);
-    run += r.hit ? nf : 0;
-    off[b + 1] = run;
+  __syncthreads();
+  int32_t run = incl - s + (wv ? wsum[wv - 1] : 0);
+  for (int64_t j = 0; j < per; ++j) {
+    if (lo + j >= B) break;
+    run += ((hits >> j) & 1) ? nf : 0;
+    off[lo + j + 1] = run;
   }
   if (t == 0) off[0] = 0;
 }
