@@ -184,6 +184,105 @@ def run_atomics(args):
             "cpu_baseline": None}, None
 
 
+def run_mesh(args):
+    """Inference-only kernels of the export and occupancy paths.
+
+    Mesh export (utils.py:183-249 via train_per_scene.py:32-41): marching
+    tetrahedra of a 256^3 FP32 signed-distance volume of the synthetic SDF
+    scene (32 spheres and 16 boxes), the whole ffields.mesh.marching_tetrahedra
+    call (count, scan, emit, key sort, vertices) timed with HIP events; the
+    count kernel (ff_mt_count, 8 B per cube: one 4 B value read per cube, its
+    neighbours' from cache, and the 4 B count written) timed alone for the
+    roofline.  Occupancy filter (models/FactorFields.py:774-775): the
+    small-object filter ff_remove_small_components on the default 200^3 grid
+    of a ball, a shell and 2 % scattered floaters.  CPU baselines: the
+    oracle's marching tetrahedra on a 96^3 crop of the same volume and the
+    oracle's scipy labelling of the same mask."""
+    import ctypes
+    import numpy as np
+    import torch
+    import ffields._lib as L
+    from ffields import workloads as W
+    from ffields.mesh import marching_tetrahedra
+    from ffields.render import remove_small_components
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    n = 256
+    scene = W.SdfScene()
+    g = torch.linspace(-1, 1, n, dtype=torch.float64)
+    vol = torch.empty((n, n, n))
+    for i in range(n):                       # slabs: the scene's SDF is evaluated on the host
+        y, z = torch.meshgrid(g, g, indexing="ij")
+        pts = torch.stack([torch.full_like(y, float(g[i])), y, z], -1).reshape(-1, 3)
+        vol[i] = scene.sdf(pts).reshape(n, n).float()
+    vol = vol.to(dev)
+
+    def timed(fn, reps):
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    verts, faces = marching_tetrahedra(vol, 0.0, inside_above=False)
+    mesh_ms = timed(lambda: marching_tetrahedra(vol, 0.0, inside_above=False), args.steps)
+    cubes = (n - 1) ** 3
+    counts = torch.empty(cubes, dtype=torch.int32, device=dev)
+    lev = ctypes.c_double(0.0)
+    count_ms = timed(lambda: L.check(L.lib().ff_mt_count(L.ptr(vol), n, n, n, lev, 0, L.ptr(counts),
+                                                          L.stream_of(vol)), "ff_mt_count"), args.steps)
+    log(f"mesh: {n}^3, {faces.shape[0]} triangles, {verts.shape[0]} vertices, {mesh_ms:.3f} ms "
+        f"(count kernel {count_ms * 1e3:.1f} us)")
+    count_gbs = 8 * cubes / (count_ms * 1e-3) / 1e9
+
+    m = 200
+    gm = np.linspace(-1, 1, m)
+    zz, yy, xx = np.meshgrid(gm, gm, gm, indexing="ij")
+    r = np.sqrt(xx * xx + yy * yy + zz * zz)
+    mask_np = (r < 0.3) | ((r > 0.6) & (r < 0.62)) | (np.random.default_rng(3).random((m, m, m)) < 0.02)
+    mask = torch.from_numpy(mask_np.astype(np.uint8)).to(dev)
+    keep = remove_small_components(mask, float(m))
+    cc_ms = timed(lambda: remove_small_components(mask, float(m)), args.steps)
+    log(f"components: {m}^3, {int(keep.sum())} of {int(mask.sum())} voxels kept, {cc_ms:.3f} ms")
+
+    out = {"metric": METRIC, "value": cubes / (mesh_ms * 1e-3), "unit": "grid cubes/s (mesh extraction)",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": mesh_ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
+           "data": "synthetic (seeded sphere/box union SDF on a 256^3 grid; ball + shell + 2% floaters mask)",
+           "config": {"workload": "mesh export: marching tetrahedra of a 256^3 SDF volume; occupancy small-object "
+                                  "filter on 200^3",
+                      "triangles": int(faces.shape[0]), "vertices": int(verts.shape[0]),
+                      "components": {"ms": cc_ms, "voxels_per_s": m ** 3 / (cc_ms * 1e-3),
+                                     "kept": int(keep.sum()), "set": int(mask.sum())}},
+           "roofline": {"bound": "hbm", "achieved": count_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": count_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "mt_count_kernel",
+                        "kernel_ms": count_ms, "bytes_per_cube": 8},
+           "cpu_baseline": None}
+    if not args.no_cpu_baseline:
+        from oracle.mesh import marching_tetrahedra as oracle_mt
+        from oracle.render import remove_small_objects
+        torch.set_num_threads(min(16, os.cpu_count() or 1))
+        crop = vol[:96, :96, :96].double().cpu()
+        t0 = time.perf_counter()
+        oracle_mt(crop, 0.0, inside_above=False)
+        mt_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        remove_small_objects(mask_np, float(m))
+        cc_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 95 ** 3 / mt_s, "unit": "grid cubes/s (mesh extraction)",
+                               "cores": torch.get_num_threads(), "kind": "port",
+                               "components_voxels_per_s": m ** 3 / cc_s,
+                               "sample": "oracle marching tetrahedra (torch CPU) on a 96^3 crop of the same volume; "
+                                         "oracle small-object filter (scipy.ndimage.label, one thread) on the same "
+                                         "200^3 mask"}
+    return out
+
+
 def nerf_setup(dev, pool_rays=1 << 20, seed_off=0):
     import torch
     from ffields import workloads as W
@@ -862,7 +961,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="nerf", choices=["sdf", "nerf", "image", "dict", "atomics"])
+    ap.add_argument("--workload", default="nerf", choices=["sdf", "nerf", "image", "dict", "atomics", "mesh"])
     ap.add_argument("--dict-images", type=int, default=1024)
     ap.add_argument("--dict-batch", type=int, default=1 << 22)
     ap.add_argument("--dict-steps", type=int, default=100, help="training steps before the timed ones")
@@ -907,6 +1006,9 @@ def main():
         if args.others:
             result["config"]["other_workloads"] = run_others(args, rank)
         print(json.dumps(result), flush=True)
+        return
+    if args.workload == "mesh":
+        print(json.dumps(run_mesh(args)), flush=True)
         return
     if args.workload == "atomics":
         result, _ = run_atomics(args)

@@ -6,8 +6,9 @@
 // level contributes one or two triangles whose vertices lie on grid edges.
 //
 //   ff_mt_count: one thread per cube, the number of triangles it emits;
-//   ff_mt_emit:  one thread per cube, at its offset in the exclusive scan of
-//                the counts: three edge keys per triangle (key = lo * N + hi
+//   ff_mt_emit:  one thread per listed cube (those with triangles), at its
+//                offset in the exclusive scan of the counts: three edge keys
+//                per triangle (key = lo * N + hi
 //                for grid points lo < hi, N the volume's point count), in an
 //                order whose normal points from the tetrahedron's inside
 //                corners to its outside ones;
@@ -63,6 +64,19 @@ __device__ __forceinline__ MtCase mt_case(int m) {
   return c;
 }
 
+// q = x / d, r = x % d with 32-bit division when both operands fit in 32 bits
+// (every volume up to 2^32 points), 64-bit otherwise.
+__device__ __forceinline__ void divmod(int64_t x, int64_t d, int64_t& q, int64_t& r) {
+  if (((uint64_t)x | (uint64_t)d) >> 32 == 0) {
+    const uint32_t q32 = (uint32_t)x / (uint32_t)d;
+    q = q32;
+    r = (int64_t)((uint32_t)x - q32 * (uint32_t)d);
+  } else {
+    q = x / d;
+    r = x - q * d;
+  }
+}
+
 struct MtGrid {
   const float* v;
   int64_t A, B, C;
@@ -74,11 +88,18 @@ struct MtGrid {
   }
 };
 
-// The cube's 8 point ids and inside flags; returns false for cubes off the grid.
-__device__ __forceinline__ bool mt_cube(const MtGrid& g, int64_t c, int64_t (&pid)[8], bool (&in)[8]) {
+// The cube's origin (i, j, k), its 8 point ids and inside flags; returns
+// false for cubes off the grid.
+__device__ __forceinline__ bool mt_cube(const MtGrid& g, int64_t c, int64_t (&ijk)[3], int64_t (&pid)[8],
+                                        bool (&in)[8]) {
   const int64_t nk = g.C - 1, nj = g.B - 1, ni = g.A - 1;
-  if (c >= ni * nj * nk) return false;
-  const int64_t k = c % nk, j = (c / nk) % nj, i = c / (nk * nj);
+  if (c < 0 || c >= ni * nj * nk) return false;
+  int64_t q, k, i, j;
+  divmod(c, nk, q, k);
+  divmod(q, nj, i, j);
+  ijk[0] = i;
+  ijk[1] = j;
+  ijk[2] = k;
 #pragma unroll
   for (int o = 0; o < 8; ++o) {
     pid[o] = g.pid(i + (o & 1), j + ((o >> 1) & 1), k + (o >> 2));
@@ -95,28 +116,37 @@ __device__ __forceinline__ int tet_mask(const bool (&in)[8], int t) {
 }
 
 __device__ __forceinline__ void mt_coords(const MtGrid& g, int64_t p, double* x) {
-  x[0] = (double)(p / (g.B * g.C));
-  x[1] = (double)((p / g.C) % g.B);
-  x[2] = (double)(p % g.C);
+  int64_t q, k, i, j;
+  divmod(p, g.C, q, k);
+  divmod(q, g.B, i, j);
+  x[0] = (double)i;
+  x[1] = (double)j;
+  x[2] = (double)k;
 }
 
-// The level crossing on the edge between grid points p < q.
-__device__ __forceinline__ void mt_vertex(const MtGrid& g, int64_t p, int64_t q, double* out) {
+// The level crossing on the edge between grid points p < q at index
+// coordinates a, b.
+__device__ __forceinline__ void mt_vertex_at(const MtGrid& g, int64_t p, int64_t q, const double* a,
+                                             const double* b, double* out) {
 #pragma clang fp contract(off)
   const double f0 = g.v[p], f1 = g.v[q];
   const double t = (g.level - f0) / (f1 - f0);
-  double a[3], b[3];
-  mt_coords(g, p, a);
-  mt_coords(g, q, b);
 #pragma unroll
   for (int k = 0; k < 3; ++k) out[k] = a[k] + t * (b[k] - a[k]);
 }
 
+__device__ __forceinline__ void mt_vertex(const MtGrid& g, int64_t p, int64_t q, double* out) {
+  double a[3], b[3];
+  mt_coords(g, p, a);
+  mt_coords(g, q, b);
+  mt_vertex_at(g, p, q, a, b, out);
+}
+
 __global__ __launch_bounds__(256) void mt_count_kernel(MtGrid g, int32_t* __restrict__ counts) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t pid[8];
+  int64_t ijk[3], pid[8];
   bool in[8];
-  if (!mt_cube(g, c, pid, in)) return;
+  if (!mt_cube(g, c, ijk, pid, in)) return;
   int n = 0;
 #pragma unroll
   for (int t = 0; t < 6; ++t) {
@@ -127,14 +157,26 @@ __global__ __launch_bounds__(256) void mt_count_kernel(MtGrid g, int32_t* __rest
   counts[c] = n;
 }
 
-__global__ __launch_bounds__(256) void mt_emit_kernel(MtGrid g, const int64_t* __restrict__ offsets,
+// One thread per listed cube (cubes[i], or cube i when the list is NULL).
+__global__ __launch_bounds__(256) void mt_emit_kernel(MtGrid g, const int64_t* __restrict__ cubes, int64_t n_cubes,
+                                                      const int64_t* __restrict__ offsets,
                                                       int64_t* __restrict__ keys) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t pid[8];
+  const int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= n_cubes) return;
+  const int64_t c = cubes ? cubes[ci] : ci;
+  int64_t ijk[3], pid[8];
   bool in[8];
-  if (!mt_cube(g, c, pid, in)) return;
+  if (!mt_cube(g, c, ijk, pid, in)) return;
   int64_t o = offsets[c];
   const int64_t N = g.A * g.B * g.C;
+  // index coordinates of the 8 corners (exact in FP64)
+  double X[8][3];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    X[q][0] = (double)(ijk[0] + (q & 1));
+    X[q][1] = (double)(ijk[1] + ((q >> 1) & 1));
+    X[q][2] = (double)(ijk[2] + (q >> 2));
+  }
   for (int t = 0; t < 6; ++t) {
     const int m = tet_mask(in, t);
     if (m == 0 || m == 15) continue;
@@ -143,8 +185,7 @@ __global__ __launch_bounds__(256) void mt_emit_kernel(MtGrid g, const int64_t* _
     double cin[3] = {0, 0, 0}, cout[3] = {0, 0, 0};
     int nin = 0;
     for (int q = 0; q < 4; ++q) {
-      double x[3];
-      mt_coords(g, pid[kTet[t][q]], x);
+      const double* x = X[kTet[t][q]];
       const bool iq = (m >> q) & 1;
       nin += iq;
       for (int a = 0; a < 3; ++a) (iq ? cin : cout)[a] += x[a];
@@ -156,10 +197,12 @@ __global__ __launch_bounds__(256) void mt_emit_kernel(MtGrid g, const int64_t* _
       int64_t key[3];
       double p[3][3];
       for (int e = 0; e < 3; ++e) {
-        const int64_t pu = pid[kTet[t][cs.e[tr][e][0]]], pv = pid[kTet[t][cs.e[tr][e][1]]];
-        const int64_t lo = pu < pv ? pu : pv, hi = pu < pv ? pv : pu;
+        const int cu = kTet[t][cs.e[tr][e][0]], cv = kTet[t][cs.e[tr][e][1]];
+        const int64_t pu = pid[cu], pv = pid[cv];
+        const bool uv = pu < pv;
+        const int64_t lo = uv ? pu : pv, hi = uv ? pv : pu;
         key[e] = lo * N + hi;
-        mt_vertex(g, lo, hi, p[e]);
+        mt_vertex_at(g, lo, hi, X[uv ? cu : cv], X[uv ? cv : cu], p[e]);
       }
       const double u[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
       const double w[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
@@ -178,8 +221,9 @@ __global__ __launch_bounds__(256) void mt_verts_kernel(MtGrid g, const int64_t* 
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t N = g.A * g.B * g.C;
-  const int64_t lo = keys[i] / N, hi = keys[i] % N;
-  if (lo < 0 || hi >= N || lo >= N) {
+  int64_t lo, hi;
+  divmod(keys[i], N, lo, hi);
+  if (keys[i] < 0 || lo >= N) {
     for (int k = 0; k < 3; ++k) verts[i * 3 + k] = __builtin_nan("");
     return;
   }
@@ -217,14 +261,17 @@ int ff_mt_count(const float* vol, int64_t A, int64_t B, int64_t C, double level,
 }
 
 int ff_mt_emit(const float* vol, int64_t A, int64_t B, int64_t C, double level, int32_t inside_above,
-               const int64_t* offsets, int64_t* keys, void* stream) {
+               const int64_t* cubes, int64_t n_cubes, const int64_t* offsets, int64_t* keys, void* stream) {
   const int st = mt_args(vol, A, B, C);
   if (st) return st;
-  if (!offsets || !keys) return FF_ERR_ARG;
   const int64_t n = (A - 1) * (B - 1) * (C - 1);
+  if (!cubes) n_cubes = n;
+  if (n_cubes < 0 || n_cubes > n) return FF_ERR_ARG;
+  if (n_cubes == 0) return FF_OK;
+  if (!offsets || !keys) return FF_ERR_ARG;
   const MtGrid g{vol, A, B, C, level, inside_above};
-  hipLaunchKernelGGL(mt_emit_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, g,
-                     offsets, keys);
+  hipLaunchKernelGGL(mt_emit_kernel, dim3((unsigned)((n_cubes + 255) / 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     cubes, n_cubes, offsets, keys);
   return mt_status();
 }
 

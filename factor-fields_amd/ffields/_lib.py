@@ -170,7 +170,7 @@ def _declare(lib):
         "ff_remove_small_components": (ctypes.c_int, [c_vp, c_i32, c_i32, c_i32, ctypes.c_double, c_vp, c_vp,
                                                        c_vp, c_vp]),
         "ff_mt_count": (ctypes.c_int, [c_vp, I64, I64, I64, ctypes.c_double, c_i32, c_vp, c_vp]),
-        "ff_mt_emit": (ctypes.c_int, [c_vp, I64, I64, I64, ctypes.c_double, c_i32, c_vp, c_vp, c_vp]),
+        "ff_mt_emit": (ctypes.c_int, [c_vp, I64, I64, I64, ctypes.c_double, c_i32, c_vp, I64, c_vp, c_vp, c_vp]),
         "ff_mt_verts": (ctypes.c_int, [c_vp, I64, I64, I64, ctypes.c_double, c_vp, I64, c_vp, c_vp]),
         "ff_composite_fwd": (ctypes.c_int, [P(Samples), I64, c_vp, c_vp, ctypes.c_int, c_f32, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp]),

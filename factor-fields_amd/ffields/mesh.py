@@ -27,10 +27,12 @@ def marching_tetrahedra(vol, level, inside_above=True):
     by the grid edge they lie on, faces [F, 3] int64 in cube order), every
     face oriented with its normal pointing from inside to outside.
 
-    Three launches and two torch steps between them: the triangle count of
-    every cube, its exclusive prefix sum (torch.cumsum), the edge keys of every
-    triangle written at its cube's offset, the distinct keys and every
-    corner's rank among them (torch.unique), and one vertex per distinct key."""
+    Three launches and torch steps between them: the triangle count of every
+    cube, its exclusive prefix sum (torch.cumsum) and the list of cubes with
+    triangles (torch.nonzero), the edge keys of every triangle written at its
+    cube's offset (one thread per listed cube), the distinct keys and every
+    corner's rank among them (torch.unique), and one vertex per distinct
+    key."""
     if not vol.is_cuda:
         raise ValueError("marching_tetrahedra runs on the GPU: pass the volume as a CUDA tensor")
     if vol.dim() != 3:
@@ -49,9 +51,10 @@ def marching_tetrahedra(vol, level, inside_above=True):
     if n_tri == 0:
         return empty
     offsets = ends - counts
+    cubes = torch.nonzero(counts).squeeze(1)                # cubes with triangles
     keys = torch.empty(3 * n_tri, dtype=torch.int64, device=dev)
-    L.check(lib.ff_mt_emit(L.ptr(v), A, B, C, lev, int(bool(inside_above)), L.ptr(offsets), L.ptr(keys), st),
-            "ff_mt_emit")
+    L.check(lib.ff_mt_emit(L.ptr(v), A, B, C, lev, int(bool(inside_above)), L.ptr(cubes), cubes.numel(),
+                           L.ptr(offsets), L.ptr(keys), st), "ff_mt_emit")
     uniq, inv = torch.unique(keys, return_inverse=True)
     verts = torch.empty((uniq.numel(), 3), dtype=torch.float64, device=dev)
     L.check(lib.ff_mt_verts(L.ptr(v), A, B, C, lev, L.ptr(uniq), uniq.numel(), L.ptr(verts), st), "ff_mt_verts")

@@ -554,10 +554,12 @@ int ff_remove_small_components(const uint8_t* mask, int32_t D, int32_t H, int32_
  *
  * ff_mt_count: counts[c] = triangles of cube c, c = (i * (B-1) + j) * (C-1)
  *   + k, for the (A-1)(B-1)(C-1) cubes.
- * ff_mt_emit: offsets[c] = exclusive prefix sum of counts; writes three edge
- *   keys per triangle at keys[3 * offsets[c] ...]: key = lo * (A B C) + hi
- *   for the edge between grid points lo < hi, ordered so that the normal
- *   points from the tetrahedron's inside corners to its outside ones.
+ * ff_mt_emit: offsets[c] = exclusive prefix sum of counts; for each cube c
+ *   of the list cubes[0 .. n_cubes) (every cube when cubes is NULL; listing
+ *   only the cubes with triangles saves the others' threads) writes three
+ *   edge keys per triangle at keys[3 * offsets[c] ...]: key = lo * (A B C)
+ *   + hi for the edge between grid points lo < hi, ordered so that the
+ *   normal points from the tetrahedron's inside corners to its outside ones.
  * ff_mt_verts: for n distinct keys, verts[i] = the FP64 point in index
  *   coordinates where the level crosses edge keys[i] (linear in the two
  *   endpoint values, from the lower point).  The caller deduplicates the
@@ -565,7 +567,7 @@ int ff_remove_small_components(const uint8_t* mask, int32_t D, int32_t H, int32_
 int ff_mt_count(const float* vol, int64_t A, int64_t B, int64_t C, double level, int32_t inside_above,
                 int32_t* counts, void* stream);
 int ff_mt_emit(const float* vol, int64_t A, int64_t B, int64_t C, double level, int32_t inside_above,
-               const int64_t* offsets, int64_t* keys, void* stream);
+               const int64_t* cubes, int64_t n_cubes, const int64_t* offsets, int64_t* keys, void* stream);
 int ff_mt_verts(const float* vol, int64_t A, int64_t B, int64_t C, double level, const int64_t* keys, int64_t n,
                 double* verts, void* stream);
 

@@ -199,8 +199,11 @@ def test_mesh_entry_points_validate_arguments():
     assert lib.ff_mt_count(P(0x1000), 4, 4, (1 << 20) + 1, lev, 1, P(0x2000), None) == L.FF_ERR_ARG
     assert lib.ff_mt_count(P(0x1000), 1 << 20, 1 << 20, 4, lev, 1, P(0x2000), None) == L.FF_ERR_ARG  # keys overflow
     assert lib.ff_mt_count(P(0x1000), 4, 4, 4, lev, 1, None, None) == L.FF_ERR_ARG
-    assert lib.ff_mt_emit(P(0x1000), 4, 4, 4, lev, 1, None, P(0x3000), None) == L.FF_ERR_ARG
-    assert lib.ff_mt_emit(P(0x1000), 4, 4, 4, lev, 1, P(0x2000), None, None) == L.FF_ERR_ARG
+    assert lib.ff_mt_emit(P(0x1000), 4, 4, 4, lev, 1, None, 0, None, P(0x3000), None) == L.FF_ERR_ARG
+    assert lib.ff_mt_emit(P(0x1000), 4, 4, 4, lev, 1, None, 0, P(0x2000), None, None) == L.FF_ERR_ARG
+    assert lib.ff_mt_emit(P(0x1000), 4, 4, 4, lev, 1, P(0x4000), 28, P(0x2000), P(0x3000), None) == L.FF_ERR_ARG
+    assert lib.ff_mt_emit(P(0x1000), 4, 4, 4, lev, 1, P(0x4000), -1, P(0x2000), P(0x3000), None) == L.FF_ERR_ARG
+    assert lib.ff_mt_emit(P(0x1000), 4, 4, 4, lev, 1, P(0x4000), 0, None, None, None) == L.FF_OK
     assert lib.ff_mt_verts(P(0x1000), 4, 4, 4, lev, None, 0, None, None) == L.FF_OK
     assert lib.ff_mt_verts(P(0x1000), 4, 4, 4, lev, P(0x2000), -1, P(0x3000), None) == L.FF_ERR_ARG
     assert lib.ff_mt_verts(P(0x1000), 4, 4, 4, lev, None, 5, P(0x3000), None) == L.FF_ERR_ARG
