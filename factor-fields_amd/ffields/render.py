@@ -18,10 +18,11 @@ import warnings
 import numpy as np
 import torch
 import torch.nn as nn
+import torch.nn.functional as F
 
 from . import _lib as L
 from . import ops
-from .ops import mlp_desc
+from .ops import mlp_desc, texel_major
 from .params import N_to_reso, cal_n_samples
 
 
@@ -204,13 +205,22 @@ class RenderMixin:
     @torch.no_grad()
     def upsample_volume_grid(self, res_target):
         """models/FactorFields.py:701-708: new render parameters for the grid
-        resolution res_target (step size and sample count).  The reference
-        also resizes the coefficient grid for its multi-scene google_objs
-        loader, which is not part of this library."""
+        resolution res_target (step size and sample count); for a
+        single-scene google_objs configuration with a grid coefficient, the
+        coefficient grid is also resized by 1.3 on every axis (trilinear,
+        align_corners=True, as the reference does).  The resized grid is a
+        new parameter, so an optimizer must be rebuilt afterwards (the
+        reference's training loop does so, train_per_scene.py:213-218)."""
         self.update_renderParams(res_target)
         if (self.cfg.dataset.get("dataset_name") == "google_objs" and self.n_scene == 1
                 and self.coeff_type == "grid"):
-            raise NotImplementedError("coefficient upsampling of the google_objs loader is not built")
+            with torch.no_grad():
+                c = self.coeffs[0].data
+                up = F.interpolate(c.contiguous(), size=None, scale_factor=1.3, align_corners=True,
+                                   mode="trilinear" if c.dim() == 5 else "bilinear")
+            self.coeffs = nn.ParameterList([nn.Parameter(texel_major(up))])
+            self.coeff_reso = list(up.shape[2:])
+            self._build_spec()
 
     def _aabb_host(self):
         """Host copy of the box (the kernels take it by value); refreshed only

@@ -121,3 +121,32 @@ def test_config_from_any_mapping():
     p = plain_cfg(c)
     assert type(p) is dict and type(p["model"]) is dict and p == {"model": {"basis_dims": [4, 2], "coeff_type": "grid"},
                                                                   "renderer": {"step_ratio": 0.5}}
+
+
+def test_upsample_google_objs_coefficient_grid():
+    """models/FactorFields.py:704-707: a single-scene google_objs model with a
+    grid coefficient resizes it by 1.3 per axis (trilinear, align_corners);
+    other configurations keep their coefficients.  (CPU: only the render
+    parameters and the resize run, no kernel.)"""
+    import torch.nn.functional as F
+    from ffields.ops import is_texel_major
+    from test_gpu_render import nerf_small_cfg
+    cfg = nerf_small_cfg()
+    cfg.dataset.dataset_name = "google_objs"
+    m = FactorFields(cfg, "cpu")
+    g = torch.Generator().manual_seed(5)
+    with torch.no_grad():
+        m.coeffs[0].copy_(torch.randn(m.coeffs[0].shape, generator=g))
+    old = m.coeffs[0].detach().clone()
+    m.upsample_volume_grid([20, 22, 24])
+    want = F.interpolate(old.contiguous(), scale_factor=1.3, align_corners=True, mode="trilinear")
+    c = m.coeffs[0]
+    assert tuple(c.shape) == tuple(want.shape) and list(m.coeff_reso) == list(want.shape[2:])
+    assert torch.allclose(c.detach(), want, rtol=0, atol=1e-6)
+    assert is_texel_major(c)
+    assert m._spec.coeff_reso_xyz == list(want.shape[2:])[::-1]
+    assert m.gridSize.tolist() == [20, 22, 24]
+    m2 = FactorFields(nerf_small_cfg(), "cpu")
+    shape = tuple(m2.coeffs[0].shape)
+    m2.upsample_volume_grid([20, 22, 24])
+    assert tuple(m2.coeffs[0].shape) == shape
