@@ -105,3 +105,29 @@ def test_export_mesh(tmp_path):
     p = torch.from_numpy(v.copy()).double()
     assert bool(((p >= lo - 1e-5) & (p <= hi + 1e-5)).all())
     assert int(f.min()) >= 0 and int(f.max()) < v.shape[0]
+
+
+def test_emit_without_cube_list():
+    """ff_mt_emit with no cube list (one thread per cube of the grid) writes
+    the same keys at the same offsets as with the list of cubes that have
+    triangles."""
+    import ctypes
+    import ffields._lib as L
+    print("[case] marching tetrahedra: emit over every cube", file=sys.stderr, flush=True)
+    vol = torch.rand((11, 9, 13), generator=torch.Generator().manual_seed(4)).cuda()
+    A, B, C = vol.shape
+    lev = ctypes.c_double(0.5)
+    counts = torch.empty((A - 1) * (B - 1) * (C - 1), dtype=torch.int32, device="cuda")
+    L.check(L.lib().ff_mt_count(L.ptr(vol), A, B, C, lev, 1, L.ptr(counts), L.stream_of(vol)))
+    ends = torch.cumsum(counts, 0, dtype=torch.int64)
+    offsets = ends - counts
+    n_tri = int(ends[-1])
+    cubes = torch.nonzero(counts).squeeze(1)
+    k_all = torch.empty(3 * n_tri, dtype=torch.int64, device="cuda")
+    k_list = torch.empty(3 * n_tri, dtype=torch.int64, device="cuda")
+    L.check(L.lib().ff_mt_emit(L.ptr(vol), A, B, C, lev, 1, None, 0, L.ptr(offsets), L.ptr(k_all),
+                               L.stream_of(vol)))
+    L.check(L.lib().ff_mt_emit(L.ptr(vol), A, B, C, lev, 1, L.ptr(cubes), cubes.numel(), L.ptr(offsets),
+                               L.ptr(k_list), L.stream_of(vol)))
+    assert n_tri > 0 and torch.equal(k_all, k_list)
+

@@ -84,11 +84,12 @@ def _spiral(n):
     return m
 
 
-@pytest.mark.parametrize("case", ["sparse", "percolating", "dense", "thin", "ragged", "all", "none", "spiral"])
+@pytest.mark.parametrize("case", ["sparse", "percolating", "dense", "thin", "ragged", "column", "all", "none",
+                                  "spiral"])
 def test_small_components_match_oracle(case):
     print(f"[case] remove_small_components: {case}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(7)
-    shape = {"thin": (1, 1, 300), "ragged": (3, 41, 17)}.get(case, (29, 23, 31))
+    shape = {"thin": (1, 1, 300), "ragged": (3, 41, 17), "column": (40, 30, 1)}.get(case, (29, 23, 31))
     if case == "spiral":
         m = _spiral(33)
     elif case == "all":
@@ -96,7 +97,7 @@ def test_small_components_match_oracle(case):
     elif case == "none":
         m = np.zeros(shape, bool)
     else:
-        p = {"sparse": 0.15, "percolating": 0.32, "dense": 0.7, "thin": 0.8, "ragged": 0.45}[case]
+        p = {"sparse": 0.15, "percolating": 0.32, "dense": 0.7, "thin": 0.8, "ragged": 0.45, "column": 0.6}[case]
         m = rng.random(shape) < p
     ms = float(np.mean(m.shape))
     out, lab, size, (r_out, r_first, r_size) = _components(m, ms)
