@@ -463,6 +463,70 @@ __device__ __forceinline__ void coding_bwd_wave(const ff_field_desc& fd, const f
   }
 }
 
+// VM coefficient lines accumulated in LDS: the three lines of length up to
+// kVmLdsLine texels (C channels each) are small enough to sum in a block's
+// LDS, [3][kVmLdsLine][C] floats, and reach the global gradients once per
+// block (vm_lines_flush) instead of with every point's atomics.
+constexpr int kVmLdsLine = 128;
+
+// The line's corners and weights are those grid_scatter_wave forms for the
+// [1, R] line table (the unit axis contributes weight 1), added with LDS
+// atomics.
+template <int C>
+__device__ __forceinline__ void vm_line_lds_add(float* line, int R, float pt, int interp, const float* grad,
+                                                bool active) {
+  if (!active) return;
+  const float p = safe_coord(unnorm_border(pt, R), R);
+  if (interp == FF_INTERP_BILINEAR) {
+    const float f0 = floorf(p);
+    const int i0 = (int)f0;
+    const float w1 = p - f0, w0 = (f0 + 1.f) - p;
+    const bool ok0 = i0 >= 0 && i0 < R && w0 != 0.f, ok1 = i0 + 1 >= 0 && i0 + 1 < R && w1 != 0.f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (ok0) atomicAdd(line + i0 * C + c, w0 * grad[c]);
+      if (ok1) atomicAdd(line + (i0 + 1) * C + c, w1 * grad[c]);
+    }
+  } else {
+    const int t = (int)rintf(p);
+    if (t < 0 || t >= R) return;
+#pragma unroll
+    for (int c = 0; c < C; ++c) atomicAdd(line + t * C + c, grad[c]);
+  }
+}
+
+// coding_bwd_wave for the VM coefficient with its lines summed in LDS
+// (lines: [3][kVmLdsLine][C], line reso <= kVmLdsLine); planes and basis as
+// coding_bwd_wave.
+template <int D, bool BF, class S, bool kB>
+__device__ __forceinline__ void coding_bwd_wave_vm_lds(const ff_field_desc& fd, const ff_grads& g, const float* x,
+                                                       const float* dbas, const float* dcof, bool active,
+                                                       float* stage, float* lines) {
+  constexpr int C = S::C;
+  if (fd.basis_kind == FF_BASIS_GRID) {
+    static_for<0, S::L>([&](auto LI) {
+      constexpr int l = decltype(LI)::value;
+      basis_level_bwd_wave<D, S::dim(l), kB>(fd, g.basis[l], l, x, dbas + S::off(l), active, stage);
+    });
+  }
+  float pts[3];
+  coeff_pts<3>(fd, x, pts);
+#pragma unroll 1
+  for (int m = 0; m < 3; ++m) {
+    float pv[C], lv[C], t[C];
+    vm_terms<C, BF>(fd, m, pts, pv, lv);
+    const int a0 = kMatMode[m][0], a1 = kMatMode[m][1], av = kVecMode[m];
+    const int Rp[2] = {fd.coeff_reso[a0], fd.coeff_reso[a1]};
+    const float pp[2] = {pts[a0], pts[a1]};
+#pragma unroll
+    for (int c = 0; c < C; ++c) t[c] = dcof[c] * lv[c];
+    grid_scatter_wave<2, C, kB>(g.coeff[m], Rp, pp, fd.coeff_interp, t, active, stage);
+#pragma unroll
+    for (int c = 0; c < C; ++c) t[c] = dcof[c] * pv[c];
+    vm_line_lds_add<C>(lines + m * kVmLdsLine * C, fd.coeff_reso[av], pts[av], fd.coeff_interp, t, active);
+  }
+}
+
 // feats = basis * coeff, with the 'none' variants of get_coding.
 template <int C, int CK>
 __device__ __forceinline__ void combine(const ff_field_desc& fd, const float* bas, const float* cof,
@@ -825,16 +889,41 @@ __global__ __launch_bounds__(64) void field_bwd_kernel(ff_field_desc fd, ff_mlp_
 // 1,200-3,300 weights read through the scalar cache (which the vector decoder
 // spills to lanes as 1,400-3,200 scalar registers).  Same loss modes, same
 // partial-row layout.
+//
+// VM coefficient (lines of up to kVmLdsLine texels): blocks of kVmBwdWaves
+// waves, one per SIMD of a CU, that sum the three coefficient lines'
+// gradients in the block's LDS and add them to the global gradients once at
+// the end (the lines are 3 x 128 x 18 floats; every point touches two texels
+// of each, which as global atomics were about a fifth of the kernel's line
+// requests).  `waves` is the number of logical waves (partial rows); blocks
+// cover ceil(waves / kVmBwdWaves) x kVmBwdWaves waves and the extra ones
+// only help to clear and flush the lines.
+constexpr int kVmBwdWaves = 4;
+
+template <int CK>
+constexpr int bwd_block_waves() { return CK == CK_VM ? kVmBwdWaves : 1; }
+
 template <int D, int CK, bool BF, class S, int O>
-__global__ __launch_bounds__(64) void field_bwd_mfma_kernel(ff_field_desc fd, ff_mlp_desc m, ff_grads g,
-                                                            const float* __restrict__ x, int64_t n,
-                                                            const float* __restrict__ dsrc, int mode,
-                                                            float loss_scale, float* __restrict__ ws,
-                                                            float* __restrict__ loss_sum) {
+__global__ __launch_bounds__(64 * bwd_block_waves<CK>()) void field_bwd_mfma_kernel(
+    ff_field_desc fd, ff_mlp_desc m, ff_grads g, const float* __restrict__ x, int64_t n,
+    const float* __restrict__ dsrc, int mode, float loss_scale, float* __restrict__ ws,
+    float* __restrict__ loss_sum, int64_t waves) {
   constexpr int C = S::C;
   static_assert(C == kMfIn, "the matrix-core decoder takes 18 features");
   constexpr int XD = XDim<D, CK>::value;
-  __shared__ __attribute__((aligned(16))) float lds[kF32LdsWords > kStageWords ? kF32LdsWords : kStageWords];
+  constexpr int NW = bwd_block_waves<CK>();
+  constexpr bool kLines = CK == CK_VM;
+  constexpr int kWaveWords = kF32LdsWords > kStageWords ? kF32LdsWords : kStageWords;
+  __shared__ __attribute__((aligned(16))) float lds_all[NW][kWaveWords];
+  __shared__ __attribute__((aligned(16))) float lines[kLines ? 3 * kVmLdsLine * C : 1];
+  const int wid = (int)(threadIdx.x >> 6);
+  float* lds = lds_all[wid];
+  const bool use_lines = kLines && fd.coeff_reso[0] <= kVmLdsLine && fd.coeff_reso[1] <= kVmLdsLine &&
+                         fd.coeff_reso[2] <= kVmLdsLine;
+  if constexpr (kLines) {
+    for (int i = threadIdx.x; i < 3 * kVmLdsLine * C; i += blockDim.x) lines[i] = 0.f;
+    __syncthreads();
+  }
   MfmaDecoderF32<O> W;
   W.load(m);
   MfmaDecoderF32Bwd<O> WB;
@@ -842,11 +931,12 @@ __global__ __launch_bounds__(64) void field_bwd_mfma_kernel(ff_field_desc fd, ff
   MfmaDecoderAcc<O> acc;
   acc.zero();
   float lsum = 0.f;
-  const int lane = threadIdx.x;
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const int64_t wave = (int64_t)blockIdx.x * NW + wid;
   const int64_t ntiles = (n + kWave - 1) / kWave;
   const float grad_scale = (float)(1.0 / ((double)n * O)) * loss_scale;
   const ff_grads gl = local_grads(g);
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  for (int64_t tile = wave; wave < waves && tile < ntiles; tile += waves) {
     const int64_t p = tile * kWave + lane;
     const bool active = p < n;
     const int64_t pc = active ? p : n - 1;
@@ -883,12 +973,29 @@ __global__ __launch_bounds__(64) void field_bwd_mfma_kernel(ff_field_desc fd, ff
         df, acc, lds);
     float db[C], dc[C];
     combine_bwd<C, CK>(fd, bas, cof, df, db, dc);
-    coding_bwd_wave<D, CK, BF, S, true>(fd, gl, xv, db, dc, active, lds);
+    if constexpr (kLines) {
+      if (use_lines)
+        coding_bwd_wave_vm_lds<D, BF, S, true>(fd, gl, xv, db, dc, active, lds, lines);
+      else
+        coding_bwd_wave<D, CK, BF, S, true>(fd, gl, xv, db, dc, active, lds);
+    } else {
+      coding_bwd_wave<D, CK, BF, S, true>(fd, gl, xv, db, dc, active, lds);
+    }
   }
-  acc.store(ws + (int64_t)blockIdx.x * (kMfH * kMfIn + kMfH + O * kMfH));
+  if (wave < waves) acc.store(ws + wave * (kMfH * kMfIn + kMfH + O * kMfH));
   if (mode != 0) {
     lsum = wave_sum(lsum);
     if (lane == 0 && loss_sum) unsafeAtomicAdd(loss_sum, lsum);
+  }
+  if constexpr (kLines) {
+    if (use_lines) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < 3 * kVmLdsLine * C; i += blockDim.x) {
+        const int mm = i / (kVmLdsLine * C), e = i - mm * kVmLdsLine * C;
+        const float v = lines[i];
+        if (e < fd.coeff_reso[kVecMode[mm]] * C && v != 0.f) unsafeAtomicAdd(gl.coeff[3 + mm] + e, v);
+      }
+    }
   }
 }
 
@@ -971,10 +1078,11 @@ template <int D, int CK, bool BF, int H, int O, int NL>
 int launch_field_bwd(const ff_field_desc& d, const ff_mlp_desc& m, const ff_grads& g, const float* x,
                      int64_t n, const float* dsrc, int mode, float loss_scale, float* ws,
                      unsigned blocks, float* loss_sum, hipStream_t s) {
-  if constexpr (NL == 2 && H == kMfH && O <= 32)
-    hipLaunchKernelGGL((field_bwd_mfma_kernel<D, CK, BF, Spec18, O>), dim3(blocks), dim3(64), 0, s, d, m, g, x, n,
-                       dsrc, mode, loss_scale, ws, loss_sum);
-  else
+  if constexpr (NL == 2 && H == kMfH && O <= 32) {
+    constexpr int NW = bwd_block_waves<CK>();
+    hipLaunchKernelGGL((field_bwd_mfma_kernel<D, CK, BF, Spec18, O>), dim3((blocks + NW - 1) / NW), dim3(64 * NW),
+                       0, s, d, m, g, x, n, dsrc, mode, loss_scale, ws, loss_sum, (int64_t)blocks);
+  } else
     hipLaunchKernelGGL((field_bwd_kernel<D, CK, BF, Spec18, H, O, NL>), dim3(blocks), dim3(64), 0, s, d,
                        m, g, x, n, dsrc, mode, loss_scale, ws, loss_sum);
   return launch_status();
