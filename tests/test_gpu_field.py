@@ -156,9 +156,16 @@ def test_field_fused(name, out_dim, layers):
 
 
 @pytest.mark.parametrize("loss", ["mse", "l1"])
-@pytest.mark.parametrize("name", ["2d_grid_bilinear", "3d_vm"])
+@pytest.mark.parametrize("name", ["2d_grid_bilinear", "3d_vm", "3d_vm_nearest", "3d_vm_long_lines"])
 def test_regression_step(name, loss):
-    m = new_model(small_cfg(**dict(CASES[name], out_dim=3 if name.startswith("2d") else 1)))
+    """The fused training pass against the oracle.  VM: lines of up to 128
+    texels are summed in LDS per block (bilinear and nearest); 3d_vm_long_lines
+    (coefficient lines of 140+ texels) takes the global-atomic path."""
+    print(f"[case] regression step {name} {loss}", file=sys.stderr, flush=True)
+    kw = dict(CASES["3d_vm"], coeff_reso=140) if name == "3d_vm_long_lines" else CASES[name]
+    m = new_model(small_cfg(**dict(kw, out_dim=3 if name.startswith("2d") else 1)))
+    if name == "3d_vm_long_lines":
+        assert max(m.vm_reso) > 128
     of = oracle_of(m).requires_grad_(True)
     n = 6000
     x = rand_points(m, n, seed=8)
