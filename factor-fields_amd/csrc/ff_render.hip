@@ -309,8 +309,9 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
   const int rounds = (cnt + 63) / 64;
   if (rounds > kMaxRounds) return;  // the sampler never produces more than 2048 per ray
   // forward pass: transmittance at the start of each round; lane r keeps round r's
+  // (a ray of one round starts at transmittance 1: nothing to compute)
   float carry = 1.f, my_carry = 1.f;
-  for (int r = 0; r < rounds; ++r) {
+  for (int r = 0; rounds > 1 && r < rounds; ++r) {
     const int j = r * 64 + lane;
     const bool act = j < cnt;
     const int64_t k = s0 + j;
