@@ -131,3 +131,23 @@ def test_emit_without_cube_list():
                                L.ptr(k_list), L.stream_of(vol)))
     assert n_tri > 0 and torch.equal(k_all, k_list)
 
+
+def test_export_mesh_matches_oracle(tmp_path):
+    """export_mesh's points and faces equal the oracle's marching tetrahedra
+    of the same dense alpha volume, mapped into the box the same way
+    (utils.py:215-218 axis order, 1 / (shape - 1) scaling)."""
+    print("[case] export mesh vs oracle", file=sys.stderr, flush=True)
+    m = make_model(seed=9)
+    torch.manual_seed(3)
+    alpha, _ = m.getDenseAlpha((24, 20, 16), times=1)
+    level = float(torch.quantile(alpha.flatten().float(), 0.7))
+    pts, faces = export_mesh(m, str(tmp_path / "f.ply"), gridSize=(24, 20, 16), level=level, times=1)
+    v_ref, f_ref = oracle_mt(alpha.cpu().double(), level, True)
+    shape = torch.tensor(alpha.shape, dtype=torch.float64)
+    box = m.aabb.cpu().double()
+    lo = torch.stack([box[0, 2], box[0, 1], box[0, 0]])
+    size = torch.stack([box[1, 2] - box[0, 2], box[1, 1] - box[0, 1], box[1, 0] - box[0, 0]])
+    want = lo + v_ref / (shape - 1) * size
+    assert torch.equal(pts.cpu(), want)
+    assert _rows(faces) == _rows(f_ref)
+
