@@ -59,3 +59,10 @@ def test_export_sdf_mesh(tmp_path):
     assert v.shape[0] == verts.shape[0] and f.shape[0] == faces.shape[0]
     assert float(abs(v).max()) <= 1.0 + 1e-6
     assert int(f.min()) >= 0 and int(f.max()) < v.shape[0]
+    # the same level set as the oracle's marching tetrahedra of the same volume
+    from oracle.mesh import marching_tetrahedra as oracle_mt
+    from ffields.sdf import marchcude_to_world
+    vol = eval_sdf(m, reso, 2.0)
+    v_ref, f_ref = oracle_mt(vol.cpu().double(), level, False)
+    assert torch.equal(verts.cpu(), marchcude_to_world(v_ref, reso))
+    assert sorted(map(tuple, faces.cpu().tolist())) == sorted(map(tuple, f_ref.tolist()))
