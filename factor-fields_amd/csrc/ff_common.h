@@ -217,6 +217,16 @@ __device__ __forceinline__ int nearest_texel(const float* pos, const int* R) {
   return ok ? t : -1;
 }
 
+// Atomic add into a gradient table in device memory.  The cast tells the
+// compiler the address is global, so it emits global_atomic_add_f32, which
+// counts only against vmcnt.  Through a generic pointer it would emit a FLAT
+// atomic, which also counts against lgkmcnt: every later wait for an LDS
+// read (lgkmcnt) would then wait for the atomic's round trip to memory too.
+typedef __attribute__((address_space(1))) float global_float;
+__device__ __forceinline__ void global_atomic_add(float* p, float v) {
+  __hip_atomic_fetch_add((global_float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ------------------------------------------------------------ XCD-local gradients
 // XCC_ID hardware register (which XCD this wave runs on).
 __device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 0xf; }

@@ -290,7 +290,7 @@ __device__ __forceinline__ void wave_scatter_serial(float* stage, float* dst, co
         acc = fmaf(W[s * K + k], G[s * d + c], acc);
         ++s;
       } while (s < 64 && T[s * K + k] == t);
-      if (acc != 0.f) unsafeAtomicAdd(dst + (int64_t)t * d + c, acc);
+      if (acc != 0.f) global_atomic_add(dst + (int64_t)t * d + c, acc);
     }
   }
   __syncthreads();
@@ -367,7 +367,7 @@ __device__ __forceinline__ void wave_scatter_batched(float* stage, float* dst, c
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if ((e[u] >> 10) > 0 && acc[u] != 0.f) unsafeAtomicAdd(dst + (int64_t)t[u] * d + c, acc[u]);
+      if ((e[u] >> 10) > 0 && acc[u] != 0.f) global_atomic_add(dst + (int64_t)t[u] * d + c, acc[u]);
   }
   wave_lds_fence();
 }
@@ -993,7 +993,7 @@ __global__ __launch_bounds__(64 * bwd_block_waves<CK>()) void field_bwd_mfma_ker
       for (int i = threadIdx.x; i < 3 * kVmLdsLine * C; i += blockDim.x) {
         const int mm = i / (kVmLdsLine * C), e = i - mm * kVmLdsLine * C;
         const float v = lines[i];
-        if (e < fd.coeff_reso[kVecMode[mm]] * C && v != 0.f) unsafeAtomicAdd(gl.coeff[3 + mm] + e, v);
+        if (e < fd.coeff_reso[kVecMode[mm]] * C && v != 0.f) global_atomic_add(gl.coeff[3 + mm] + e, v);
       }
     }
   }
