@@ -88,13 +88,19 @@ def run_sdf(args):
     pool_n = 4 * n
     pts, sdf = scene.training_pool(pool_n)
     pts, sdf = pts.to(dev), sdf.to(dev)
+    if args.sdf_order == "morton":
+        perm = W.morton_order(pts, cfg.dataset.aabb)
+        pts, sdf = pts[perm], sdf[perm]
     opt = FusedAdam(model.get_optparam_groups(cfg.training.lr_small, cfg.training.lr_large),
                     betas=(0.9, 0.99))
     gen = torch.Generator(device=dev).manual_seed(W.BASE_SEED + 1)
     loss_sum = torch.zeros((), device=dev)
 
     def step(ev=None):
-        idx = torch.randint(0, pool_n, (n,), device=dev, generator=gen)
+        if args.sdf_order == "morton":
+            idx = W.sorted_batch(pool_n, n, gen, dev)
+        else:
+            idx = torch.randint(0, pool_n, (n,), device=dev, generator=gen)
         x, t = pts[idx], sdf[idx]
         loss_sum.zero_()
         if ev is not None:
@@ -137,7 +143,7 @@ def run_sdf(args):
         "data": "synthetic (seeded sphere/box union SDF)",
         "config": {"workload": "sdf_regression_vm: 2^20 pts/step, 3-D grid basis [4,4,4,2,2,2]@[32..128] "
                                "sawtooth, VM coeff 3x128^2+3x128 (18 ch), MLP 18-64-1, L1, Adam",
-                   "batch_points": n, "params": model.n_parameters(), "final_l1": final_l1},
+                   "batch_points": n, "batch_order": args.sdf_order, "params": model.n_parameters(), "final_l1": final_l1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("field_bwd_mfma_kernel (sdf)"),
                      "kernel": "field_bwd_mfma_kernel (fused forward+loss+backward)",
@@ -973,6 +979,8 @@ def main():
     ap.add_argument("--dict-order", default="cell", choices=["cell", "random"],
                     help="batch order of the dictionary image set (same distribution)")
     ap.add_argument("--nerf-form", default="both", choices=["both", "plain", "proposal"])
+    ap.add_argument("--sdf-order", default="morton", choices=["morton", "random"],
+                    help="SDF batches: sorted draws from a Morton-ordered pool, or random draw order")
     ap.add_argument("--nerf-pool", type=int, default=1 << 20, help="ground-truth rays the batches are drawn from")
     ap.add_argument("--image-iters", type=int, default=2000)
     ap.add_argument("--nerf-pretrain", type=int, default=1000)

@@ -198,6 +198,46 @@ def pixel_coords(h, w, device="cpu"):
 
 
 # ---------------------------------------------------------------- 3D SDF
+def morton_order(pts, aabb, bits=10):
+    """Permutation that sorts points [N, 3] by the Morton (Z-order) key of
+    their cell in a 2^bits-per-axis grid over the box aabb [2, 3].
+
+    A batch drawn from a pool in this order with sorted indices
+    (`sorted_batch`) visits the field's texels in spatial runs: the
+    backward's run-merging scatter then adds one run per texel instead of
+    one value per point.  For the 2^20-point SDF batch (4x pool, VM
+    coefficient, six basis levels) this halves the atomic line requests
+    (46.6 -> 23.0 per point, counted on the CPU through the scatter's run
+    rules).  The losses are means and the gradients sums over the batch, so
+    the order changes only the FP32 summation order."""
+    import torch
+    lo = torch.as_tensor(aabb[0], dtype=torch.float32, device=pts.device)
+    hi = torch.as_tensor(aabb[1], dtype=torch.float32, device=pts.device)
+    side = (1 << bits)
+    q = ((pts.float() - lo) / (hi - lo) * side).floor().clamp(0, side - 1).long()
+    key = torch.zeros(len(pts), dtype=torch.long, device=pts.device)
+    for b in range(bits):
+        for a in range(3):
+            key |= ((q[:, a] >> b) & 1) << (3 * b + a)
+    return torch.sort(key, stable=True).indices
+
+
+def sorted_batch(pool_n, n, generator, device):
+    """n indices drawn uniformly with replacement from [0, pool_n), in
+    ascending order, so a batch of a Morton-ordered pool keeps the pool's
+    order.  No sort: the sorted values of n independent U(0,1) draws have the
+    distribution of the normalised partial sums S_k / S_{n+1} of n + 1
+    independent Exp(1) spacings (the Renyi representation of uniform order
+    statistics), and floor(u * pool_n) is monotone, so the indices are
+    distributed as a sorted draw of n uniform integers.  The partial sums are
+    FP64."""
+    import torch
+    e = torch.empty(n + 1, dtype=torch.float64, device=device).exponential_(generator=generator)
+    c = torch.cumsum(e, 0)
+    u = c[:n] * (pool_n / c[n])
+    return u.floor_().clamp_(max=pool_n - 1).to(torch.int32)
+
+
 class SdfScene:
     """Union of 32 spheres (r U[0.05,0.25]) and 16 axis-aligned boxes
     (half-extents U[0.05,0.3]) with centres U[-1,1]^3; exact SDF."""
@@ -251,7 +291,8 @@ class SdfScene:
         return pts + torch.randn(pts.shape, generator=g, dtype=torch.float64) * 0.01
 
     def training_pool(self, n):
-        """n points (half uniform in [-1,1]^3, half near-surface) and their SDF."""
+        """n points (half uniform in [-1,1]^3, half near-surface) and their SDF
+        (in generation order; `morton_order` gives the scatter-friendly one)."""
         g = self.g
         nu = n // 2
         uni = torch.rand((nu, 3), generator=g, dtype=torch.float64) * 2 - 1

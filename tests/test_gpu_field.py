@@ -9,7 +9,7 @@ import sys
 import pytest
 import torch
 
-from helpers import assert_close, new_model, oracle_of, rand_points, small_cfg
+from helpers import assert_close, assert_close_fp64, new_model, oracle_of, rand_points, small_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -182,6 +182,40 @@ def test_regression_step(name, loss):
     for l, (w, b) in enumerate(of.mlp):
         lin = m.linear_mat.backbone[l]
         assert_close(lin.weight.grad, w.grad, GRAD_RTOL, f"W{l}", atol_frac=1e-5)
+
+
+def test_regression_step_morton_batch():
+    """The SDF workload's model (VM 128, six basis levels) on its batch order:
+    a sorted draw from a Morton-ordered pool, with repeats (pool as large as
+    the batch), puts long runs of equal texels in every wave, so the scatter
+    sums runs of many points before each atomic.  Gradients against the FP64
+    oracle, within the FP32 oracle's own distance from it."""
+    from ffields import workloads as W
+    print("[case] regression step, SDF model, Morton batch", file=sys.stderr, flush=True)
+    cfg = W.sdf_cfg()
+    m = new_model(cfg, seed=2)
+    n = 1 << 15
+    pts, sdf = W.SdfScene().training_pool(n)
+    perm = W.morton_order(pts, cfg.dataset.aabb)
+    pts, sdf = pts[perm], sdf[perm]
+    idx = W.sorted_batch(n, n, torch.Generator().manual_seed(4), "cpu").long()
+    assert len(torch.unique(idx)) < n             # repeated points: runs across every corner slot
+    x, t = pts[idx], sdf[idx]
+    ls = m.regression_step(x.cuda(), t.cuda(), loss="l1")
+    ofs = []
+    for dt in (torch.float32, torch.float64):
+        of = oracle_of(m, dtype=dt).requires_grad_(True)
+        e = of.field(x.to(dt)) - t.to(dt)
+        e.abs().mean().backward()
+        ofs.append(of)
+    s = (ofs[1].field(x.double()) - t.double()).abs().sum().item()
+    assert abs(ls.item() - s) <= 1e-4 * abs(s)
+    for k, (tb, o32, o64) in enumerate(zip(list(m.basises) + list(m.coeffs), ofs[0].basises + ofs[0].coeffs,
+                                           ofs[1].basises + ofs[1].coeffs)):
+        assert_close_fp64(tb.grad.contiguous(), o32.grad, o64.grad, GRAD_RTOL, f"table {k}", atol_frac=1e-5)
+    for l, ((w32, _), (w64, _)) in enumerate(zip(ofs[0].mlp, ofs[1].mlp)):
+        assert_close_fp64(m.linear_mat.backbone[l].weight.grad, w32.grad, w64.grad, GRAD_RTOL, f"W{l}",
+                          atol_frac=1e-5)
 
 
 BF16_CASES = {

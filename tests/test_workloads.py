@@ -1,5 +1,6 @@
 """Synthetic workload generators: the batched image-set generator reproduces
 the single-image generator exactly (same draws, same accumulation order)."""
+import numpy as np
 import torch
 
 from ffields import workloads as W
@@ -62,3 +63,45 @@ def test_ordered_batch_distribution():
     g3 = torch.Generator().manual_seed(3)
     assert torch.equal(i2, torch.randint(0, N, (16,), generator=g3))
     assert torch.equal(p2, torch.randint(0, P, (16,), generator=g3))
+
+
+def _morton_key(q, bits):
+    """Bit-by-bit Z-order key of integer cell coordinates (x, y, z)."""
+    k = 0
+    for b in range(bits):
+        for a in range(3):
+            k |= ((int(q[a]) >> b) & 1) << (3 * b + a)
+    return k
+
+
+def test_morton_order_sorts_by_z_order_key():
+    g = torch.Generator().manual_seed(3)
+    p = torch.rand((500, 3), generator=g) * 4 - 2          # box [-2, 2]^3, some points on its faces
+    p[:5] = torch.tensor([[-2.0, -2, -2], [2, 2, 2], [2, -2, 0], [0, 0, 0], [-2, 2, 2]])
+    bits = 4
+    perm = W.morton_order(p, [[-2, -2, -2], [2, 2, 2]], bits=bits)
+    assert sorted(perm.tolist()) == list(range(500))
+    q = ((p + 2) / 4 * (1 << bits)).floor().clamp(0, (1 << bits) - 1).long()
+    keys = [_morton_key(q[i], bits) for i in perm.tolist()]
+    assert keys == sorted(keys)
+    # ties keep the pool order (stable)
+    for a, b in zip(perm.tolist(), perm.tolist()[1:]):
+        if _morton_key(q[a], bits) == _morton_key(q[b], bits):
+            assert a < b
+
+
+def test_sorted_batch_is_a_sorted_uniform_draw():
+    """Ascending, in range, and each order statistic distributed as that of
+    sorted torch.randint draws (two-sample KS, fixed seeds)."""
+    from scipy import stats
+    g = torch.Generator().manual_seed(0)
+    idx = W.sorted_batch(4000, 100000, g, "cpu")
+    assert idx.dtype == torch.int32 and bool((idx[1:] >= idx[:-1]).all())
+    assert int(idx.min()) >= 0 and int(idx.max()) < 4000
+    assert stats.chisquare(np.bincount(idx.numpy(), minlength=4000)).pvalue > 1e-3
+    g = torch.Generator().manual_seed(1)
+    a = np.stack([W.sorted_batch(100, 5, g, "cpu").numpy() for _ in range(4000)])
+    gr = torch.Generator().manual_seed(2)
+    b = np.sort(torch.randint(0, 100, (4000, 5), generator=gr).numpy(), 1)
+    for k in range(5):
+        assert stats.ks_2samp(a[:, k], b[:, k]).pvalue > 1e-3, k
