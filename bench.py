@@ -391,6 +391,7 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
     t0 = time.perf_counter()
     for k in range(timed_steps):
         step(k)
+    t_enq = time.perf_counter() - t0
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     kern = model.kernel_events
@@ -409,7 +410,8 @@ def nerf_train(form, dev, pool, pool_rgb, test_rays, test_rgb, timed_steps, pret
     bps, G = nerf_bytes_per_sample(model, 2 if prop else 4)
     samples_per_launch = samples / max(timed_steps, 1)
     res = {"rays_per_s": timed_steps * B / dt, "field_pts_per_s": samples / dt,
-           "ms_per_step": dt / timed_steps * 1e3, "samples_per_ray": samples / (timed_steps * B),
+           "ms_per_step": dt / timed_steps * 1e3, "host_enqueue_ms_per_step": t_enq / timed_steps * 1e3,
+           "samples_per_ray": samples / (timed_steps * B),
            "ns_per_sample": dt / max(samples, 1) * 1e9,
            "test_psnr": -10 * math.log10(mse), "proposal_active": proposal_on(),
            "fallback_at_step": state["fallback_at"],

@@ -170,6 +170,7 @@ class FactorFields(RenderMixin, nn.Module):
             L.check(L.lib().ff_cast_bf16(L.ptr(t), L.ptr(s), t.numel(), L.stream_of(t)), "ff_cast_bf16")
             self._shadows.append(s)
         self._shadow_src = [(t, t._version) for t in b + c]
+        self._shadow_map = {id(t): (t, s) for t, s in zip(b + c, self._shadows)}
 
     def _shadows_current(self, tabs):
         """True while every FP32 master is the tensor, at the version, its BF16
@@ -186,11 +187,13 @@ class FactorFields(RenderMixin, nn.Module):
         return True
 
     def shadow_of(self, p):
-        b, c = self._tables()
-        for t, s in zip(b + c, self._shadows or []):
-            if t is p:
-                return s
-        return None
+        """The BF16 copy cast from table p, or None.  A table that replaced p
+        since the last refresh (shrink, upsample) has no copy until the next
+        read refreshes them, so a write through this lookup never lands in a
+        copy of another tensor."""
+        m = self.__dict__.get("_shadow_map")
+        e = m.get(id(p)) if m else None
+        return e[1] if e is not None and e[0] is p else None
 
     def _build_spec(self):
         cfg = self.cfg

@@ -57,6 +57,10 @@ class FusedAdam(torch.optim.Optimizer):
                 buckets.setdefault(key, []).append(t)
         for (step, b1, b2, eps, dev), ts in buckets.items():
             arr = (L.AdamTensor * len(ts))(*ts)
+            if dev.index is None or dev.index == torch.cuda.current_device():
+                L.check(L.lib().ff_adam_multi(arr, len(ts), b1, b2, eps, step, 1 if zero_grad else 0,
+                                              L.stream_of()), "ff_adam_multi")
+                continue
             with torch.cuda.device(dev):
                 L.check(L.lib().ff_adam_multi(arr, len(ts), b1, b2, eps, step, 1 if zero_grad else 0,
                                               L.stream_of()), "ff_adam_multi")

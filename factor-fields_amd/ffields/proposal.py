@@ -56,9 +56,8 @@ class ProposalNet(nn.Module):
         return self._shadow if (p is self.grid and self.table_dtype == "bf16") else None
 
     def desc(self):
-        d = L.ProposalDesc()
-        d.reso[0] = d.reso[1] = d.reso[2] = self.reso
-        d.table_dtype = L.BF16 if self.table_dtype == "bf16" else L.F32
+        """The ProposalDesc, rebuilt only when a pointer or scalar it holds
+        changed since the last call."""
         if self.table_dtype == "bf16":
             # the grid was written through torch since the copy was cast (its
             # version moved, or it was replaced): cast again; FusedAdam keeps
@@ -66,7 +65,21 @@ class ProposalNet(nn.Module):
             src, ver = self._shadow_src
             if src is not self.grid or src._version != ver:
                 self.refresh_shadow()
-        d.grid = (self._shadow if self.table_dtype == "bf16" else self.grid).data_ptr()
+        grid = (self._shadow if self.table_dtype == "bf16" else self.grid).data_ptr()
+        key = (grid, self.mlp[0].weight.data_ptr(), self.mlp[0].bias.data_ptr(), self.mlp[1].weight.data_ptr(),
+               self.reso, self.table_dtype, tuple(map(tuple, self._aabb_list)), self.density_shift,
+               self.distance_scale, self.near, self.n_fine)
+        c = self.__dict__.get("_desc_cache")
+        if c is None or c[0] != key:
+            c = (key, self._build_desc(grid))
+            self.__dict__["_desc_cache"] = c
+        return c[1]
+
+    def _build_desc(self, grid):
+        d = L.ProposalDesc()
+        d.reso[0] = d.reso[1] = d.reso[2] = self.reso
+        d.table_dtype = L.BF16 if self.table_dtype == "bf16" else L.F32
+        d.grid = grid
         d.w0 = self.mlp[0].weight.data_ptr()
         d.b0 = self.mlp[0].bias.data_ptr()
         d.w1 = self.mlp[1].weight.data_ptr()

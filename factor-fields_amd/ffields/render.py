@@ -266,10 +266,18 @@ class RenderMixin:
         return h
 
     def _field_structs(self):
+        """(field, decoder) descriptors, rebuilt only when the spec object or
+        a table or weight pointer changed since the last call (building the
+        ctypes structs costs tens of microseconds per call)."""
         tabs = self._read_tables()
-        d = self._spec.desc(*self._split(tabs))
-        m = mlp_desc(self._layers())
-        return d, m
+        layers = self._layers()
+        key = (tuple(t.data_ptr() for t in tabs),
+               tuple(x.data_ptr() for lw in layers for x in lw if x is not None))
+        c = self.__dict__.get("_struct_cache")
+        if c is None or c[0] is not self._spec or c[1] != key:
+            c = (self._spec, key, self._spec.desc(*self._split(tabs)), mlp_desc(layers))
+            self.__dict__["_struct_cache"] = c
+        return c[2], c[3]
 
     def render_buffers(self, B, Ns):
         """Packed-sample and per-sample buffers for B rays x Ns candidates,

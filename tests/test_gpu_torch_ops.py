@@ -62,3 +62,16 @@ def test_compiled_training_step_matches_eager():
     assert torch.allclose(l0, l1, rtol=1e-6, atol=0)
     for a, b in zip(g0, g1):
         assert torch.allclose(a, b, rtol=1e-4, atol=1e-6 * float(a.abs().max()))
+
+
+def test_stream_of_is_the_current_stream():
+    """The entry points run on torch's current stream of the tensor's device,
+    also inside a stream context."""
+    import ffields._lib as L
+    t = torch.zeros(4, device="cuda")
+    assert (L.stream_of(t).value or 0) == torch.cuda.current_stream().cuda_stream
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        assert (L.stream_of(t).value or 0) == s.cuda_stream
+        assert (L.stream_of().value or 0) == s.cuda_stream
+    assert (L.stream_of().value or 0) == torch.cuda.current_stream().cuda_stream
