@@ -95,6 +95,7 @@ def run_sdf(args):
                     betas=(0.9, 0.99))
     gen = torch.Generator(device=dev).manual_seed(W.BASE_SEED + 1)
     loss_sum = torch.zeros((), device=dev)
+    x_last = [None]
 
     def step(ev=None):
         if args.sdf_order == "morton":
@@ -102,6 +103,7 @@ def run_sdf(args):
         else:
             idx = torch.randint(0, pool_n, (n,), device=dev, generator=gen)
         x, t = pts[idx], sdf[idx]
+        x_last[0] = x
         loss_sum.zero_()
         if ev is not None:
             ev[0].record()
@@ -128,6 +130,22 @@ def run_sdf(args):
     final_l1 = loss_sum.item() / n
     bpp, G = sdf_bytes_per_point(model)
     achieved = bpp * n / (kern_ms * 1e-3) / 1e9
+    # the forward gather kernel alone (field_fwd_kernel: SDF evaluation,
+    # FactorFields.forward without autograd) on one batch of the same order
+    fx = x_last[0]
+    with torch.no_grad():
+        for _ in range(3):
+            model.field(fx)
+        torch.cuda.synchronize()
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        f0.record()
+        for _ in range(20):
+            model.field(fx)
+        f1.record()
+        torch.cuda.synchronize()
+    fwd_ms = f0.elapsed_time(f1) / 20
+    fwd_bpp = 12 + 4 * G + 4                            # coordinates, gathered values, output
+    fwd_gbs = fwd_bpp * n / (fwd_ms * 1e-3) / 1e9
     result = {
         "metric": METRIC,
         "value": args.steps * n / dt,
@@ -148,6 +166,9 @@ def run_sdf(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("field_bwd_mfma_kernel (sdf)"),
                      "kernel": "field_bwd_mfma_kernel (fused forward+loss+backward)",
                      "kernel_ms": kern_ms, "bytes_per_point": bpp, "gathered_values_per_point": G},
+        "forward_gather": {"kernel": "field_fwd_kernel (forward only, one batch of the same order)",
+                           "bound": "hbm", "kernel_ms": fwd_ms, "bytes_per_point": fwd_bpp, "achieved": fwd_gbs,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fwd_gbs / HBM_PEAK_GBS},
     }
     return result, model
 
