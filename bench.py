@@ -644,6 +644,20 @@ def run_dict(args):
             r["pts_per_s"] = timed * n / dt
             r["ms_per_step"] = dt / timed * 1e3
             r["train_kernel_ms"] = sum(a.elapsed_time(b) for a, b in evs) / timed
+        if timed and form == "bf16":
+            # the forward gather kernel alone on one batch of the same order
+            fx, _ = batch(gen)
+            with torch.no_grad():
+                for _ in range(3):
+                    model.field(fx)
+                torch.cuda.synchronize()
+                f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                f0.record()
+                for _ in range(10):
+                    model.field(fx)
+                f1.record()
+                torch.cuda.synchronize()
+            r["forward_kernel_ms"] = f0.elapsed_time(f1) / 10
         res[form] = r
         log(f"dict {form}: late PSNR mean {r['psnr_late_mean']:.3f} dB over {len(scores)} evaluations"
             + (f", {r['pts_per_s']:.3e} pts/s" if timed else ""))
@@ -664,6 +678,12 @@ def run_dict(args):
                         "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("dict_bwd_kernel"),
                         "kernel": "dict_bwd_kernel (+ its decoder-gradient reduction)",
                         "kernel_ms": b["train_kernel_ms"], "bytes_per_point": bpp}}
+    fwd_bpp = 12 + 2 * 1152 + 12                        # coordinates, gathered BF16 values, rgb
+    fwd_gbs = fwd_bpp * n / (b["forward_kernel_ms"] * 1e-3) / 1e9
+    out["forward_gather"] = {"kernel": "dict_fwd_kernel (forward only, one batch of the same order)",
+                             "bound": "hbm", "kernel_ms": b["forward_kernel_ms"], "bytes_per_point": fwd_bpp,
+                             "achieved": fwd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": fwd_gbs / HBM_PEAK_GBS}
     if "fp32" in res:
         out["config"]["psnr_gap_db"] = res["fp32"]["psnr_late_mean"] - b["psnr_late_mean"]
         out["config"]["bf16_forward_cost_db"] = res["fp32"].get("bf16_forward_cost_db")
