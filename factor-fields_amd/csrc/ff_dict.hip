@@ -65,6 +65,9 @@ constexpr int kDH = 64;   // hidden units
 constexpr int kDPx = 16;  // pixels per wave tile
 constexpr int kDMt = 9;   // 16-channel tiles of features (tile 9 carries the bias column)
 constexpr int kDWaves = 4;
+// Blocks per CU of the forward kernel (its waves per SIMD): a register bound
+// the compiler meets with some spilling (232 B per lane for BF16).
+constexpr int kDFwdBlocksPerCu = 2;
 constexpr int kDOMax = 4;
 
 __device__ __forceinline__ int dlevel(int mt) { return mt < 2 ? 0 : (mt < 4 ? 1 : (mt < 6 ? 2 : mt - 3)); }
@@ -366,9 +369,8 @@ __device__ __forceinline__ void load_a2(const ff_mlp_desc& m, DV<BF> (&a2)[2]) {
 }
 
 template <bool BF>
-__global__ __launch_bounds__(256) void dict_fwd_kernel(ff_field_desc fd, ff_mlp_desc m,
-                                                       const float* __restrict__ x, int64_t n,
-                                                       float* __restrict__ out) {
+__device__ __forceinline__ void dict_fwd_body(const ff_field_desc& fd, const ff_mlp_desc& m,
+                                              const float* __restrict__ x, int64_t n, float* __restrict__ out) {
   __shared__ std::conditional_t<BF, DictW, DictW1F> W;
   if constexpr (BF)
     load_dict_weights(m, W);
@@ -397,6 +399,21 @@ __global__ __launch_bounds__(256) void dict_fwd_kernel(ff_field_desc fd, ff_mlp_
       if (active && o < O) out[p * O + o] = c2[i];
     }
   }
+}
+
+// The forward kernels: BF16 tables at kDFwdBlocksPerCu blocks per CU; the
+// FP32 form keeps one block per CU (its working set would spill 888 B per
+// lane at two).
+__global__ __launch_bounds__(256, kDFwdBlocksPerCu) void dict_fwd_kernel_bf16(ff_field_desc fd, ff_mlp_desc m,
+                                                                          const float* __restrict__ x,
+                                                                          int64_t n, float* __restrict__ out) {
+  dict_fwd_body<true>(fd, m, x, n, out);
+}
+
+__global__ __launch_bounds__(256) void dict_fwd_kernel_f32(ff_field_desc fd, ff_mlp_desc m,
+                                                           const float* __restrict__ x, int64_t n,
+                                                           float* __restrict__ out) {
+  dict_fwd_body<false>(fd, m, x, n, out);
 }
 
 // Fused backward (mode 0: d_out given in dsrc) or regression training pass
@@ -717,6 +734,14 @@ unsigned dict_blocks(int64_t n) {
   return (unsigned)(need < cap ? (need > 0 ? need : 1) : cap);
 }
 
+// The forward kernel: kDFwdBlocksPerCu blocks per CU.
+unsigned dict_fwd_blocks(int64_t n) {
+  const int64_t tiles = (n + kDPx - 1) / kDPx;
+  const int64_t need = (tiles + kDWaves - 1) / kDWaves;
+  const int64_t cap = kDFwdBlocksPerCu * dict_cu();
+  return (unsigned)(need < cap ? (need > 0 ? need : 1) : cap);
+}
+
 int dict_partial(int O) { return kDH * kDC + kDH + O * kDH; }
 
 }  // namespace
@@ -750,9 +775,9 @@ size_t dict_workspace_bytes(const ff_field_desc& d, const ff_mlp_desc& m, int64_
 int launch_dict_fwd(const ff_field_desc& d, const ff_mlp_desc& m, const float* x, int64_t n, float* out,
                     hipStream_t s) {
   if (d.table_dtype == FF_BF16)
-    hipLaunchKernelGGL(dict_fwd_kernel<true>, dim3(dict_blocks(n)), dim3(64 * kDWaves), 0, s, d, m, x, n, out);
+    hipLaunchKernelGGL(dict_fwd_kernel_bf16, dim3(dict_fwd_blocks(n)), dim3(64 * kDWaves), 0, s, d, m, x, n, out);
   else
-    hipLaunchKernelGGL(dict_fwd_kernel<false>, dim3(dict_blocks(n)), dim3(64 * kDWaves), 0, s, d, m, x, n, out);
+    hipLaunchKernelGGL(dict_fwd_kernel_f32, dim3(dict_blocks(n)), dim3(64 * kDWaves), 0, s, d, m, x, n, out);
   return hipGetLastError() == hipSuccess ? FF_OK : FF_ERR_LAUNCH;
 }
 
