@@ -993,7 +993,7 @@ def run_others(args, rank):
         try:
             r = run()
             keep = {k: r[k] for k in ("value", "unit", "ms_per_step", "steps", "warmup", "dtype", "roofline",
-                                      "config") if k in r}
+                                      "forward_gather", "config") if k in r}
             if rank == 0 and not args.no_cpu_baseline:
                 log(f"cpu baseline: {name}")
                 keep["cpu_baseline"] = cpu(args)
