@@ -167,18 +167,14 @@ def image_set_batch(n, n_images, size, generator, device, order=None):
 
     order=None: independent draws in draw order (randint, as the reference).
     order=cell_order(...): the same distribution, delivered sorted by image
-    and coefficient cell.  The sorted values of n independent uniform draws
-    on [0, K) are floor(K * S_i / S_{n+1}) with S_i the partial sums of n + 1
-    exponential variates (order statistics of uniforms), so no sort runs."""
+    and coefficient cell: a sorted draw of n keys on [0, n_images * P)
+    (`sorted_batch`, no sort runs) split into image and cell-ordered pixel."""
     P = size * size
     if order is None:
         img = torch.randint(0, n_images, (n,), device=device, generator=generator)
         pix = torch.randint(0, P, (n,), device=device, generator=generator)
         return img, pix
-    K = n_images * P
-    e = torch.empty(n + 1, dtype=torch.float64, device=device).exponential_(generator=generator)
-    s = torch.cumsum(e, 0)
-    key = torch.floor(s[:n] * (K / s[n])).long().clamp_(0, K - 1)
+    key = sorted_batch(n_images * P, n, generator, device).long()
     return key // P, order[key % P]
 
 
@@ -230,12 +226,12 @@ def sorted_batch(pool_n, n, generator, device):
     independent Exp(1) spacings (the Renyi representation of uniform order
     statistics), and floor(u * pool_n) is monotone, so the indices are
     distributed as a sorted draw of n uniform integers.  The partial sums are
-    FP64."""
+    FP64; the indices int32 (int64 for pools beyond 2^31 - 1)."""
     import torch
     e = torch.empty(n + 1, dtype=torch.float64, device=device).exponential_(generator=generator)
     c = torch.cumsum(e, 0)
     u = c[:n] * (pool_n / c[n])
-    return u.floor_().clamp_(max=pool_n - 1).to(torch.int32)
+    return u.floor_().clamp_(max=pool_n - 1).to(torch.int32 if pool_n <= 2 ** 31 - 1 else torch.int64)
 
 
 class SdfScene:

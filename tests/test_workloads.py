@@ -105,3 +105,10 @@ def test_sorted_batch_is_a_sorted_uniform_draw():
     b = np.sort(torch.randint(0, 100, (4000, 5), generator=gr).numpy(), 1)
     for k in range(5):
         assert stats.ks_2samp(a[:, k], b[:, k]).pvalue > 1e-3, k
+
+
+def test_sorted_batch_large_pool_uses_int64():
+    g = torch.Generator().manual_seed(0)
+    idx = W.sorted_batch(3 * 2 ** 31, 1000, g, "cpu")
+    assert idx.dtype == torch.int64 and int(idx.max()) < 3 * 2 ** 31 and bool((idx[1:] >= idx[:-1]).all())
+    assert int(idx.max()) > 2 ** 31                   # the upper part of the pool is reached
