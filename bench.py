@@ -678,12 +678,13 @@ def run_dict(args):
                         "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("dict_bwd_kernel"),
                         "kernel": "dict_bwd_kernel (+ its decoder-gradient reduction)",
                         "kernel_ms": b["train_kernel_ms"], "bytes_per_point": bpp}}
-    fwd_bpp = 12 + 2 * 1152 + 12                        # coordinates, gathered BF16 values, rgb
-    fwd_gbs = fwd_bpp * n / (b["forward_kernel_ms"] * 1e-3) / 1e9
-    out["forward_gather"] = {"kernel": "dict_fwd_kernel (forward only, one batch of the same order)",
-                             "bound": "hbm", "kernel_ms": b["forward_kernel_ms"], "bytes_per_point": fwd_bpp,
-                             "achieved": fwd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": fwd_gbs / HBM_PEAK_GBS}
+    if "forward_kernel_ms" in b:
+        fwd_bpp = 12 + 2 * 1152 + 12                    # coordinates, gathered BF16 values, rgb
+        fwd_gbs = fwd_bpp * n / (b["forward_kernel_ms"] * 1e-3) / 1e9
+        out["forward_gather"] = {"kernel": "dict_fwd_kernel_bf16 (forward only, one batch of the same order)",
+                                 "bound": "hbm", "kernel_ms": b["forward_kernel_ms"], "bytes_per_point": fwd_bpp,
+                                 "achieved": fwd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": fwd_gbs / HBM_PEAK_GBS}
     if "fp32" in res:
         out["config"]["psnr_gap_db"] = res["fp32"]["psnr_late_mean"] - b["psnr_late_mean"]
         out["config"]["bf16_forward_cost_db"] = res["fp32"].get("bf16_forward_cost_db")
