@@ -278,6 +278,10 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // the ray's sample range, loaded beside the colour gradient's inputs (not
+  // after the loss barrier: one memory round trip fewer per ray)
+  const int64_t s0 = b < B ? s.ray_off[b] : 0;
+  const int cnt = b < B ? (int)(s.ray_off[b + 1] - s0) : 0;
   // dL/d rgb_map, then through the clamp
   float g[3];
   float lsum = 0.f;
@@ -303,8 +307,6 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
     if (threadIdx.x == 0) unsafeAtomicAdd(loss_sum, (part[0] + part[1]) + (part[2] + part[3]));
   }
   if (b >= B) return;
-  const int64_t s0 = s.ray_off[b];
-  const int cnt = s.ray_off[b + 1] - (int)s0;
   const float gbg = white_bg ? (g[0] + g[1]) + g[2] : 0.f;
   const int rounds = (cnt + 63) / 64;
   if (rounds > kMaxRounds) return;  // the sampler never produces more than 2048 per ray
