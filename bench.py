@@ -12,8 +12,10 @@ bounded sizes and are nested in the same line under config.other_workloads;
 
 Every leg reports its dominant kernel's roofline fraction and a CPU baseline
 (the oracle, the reference's ATen ops on the host cores, on a bounded sample
-of the same workload).  Prints one JSON line (rank 0).  Only one GPU is used;
---gpus N > 1 exits.
+of the same workload); the SDF and dictionary legs also time their forward
+gather kernel alone (forward_gather).  SDF batches are sorted draws from a
+Morton-ordered pool by default (--sdf-order random: draw order).  Prints one
+JSON line (rank 0).  Only one GPU is used; --gpus N > 1 exits.
 """
 
 import argparse
